@@ -1,0 +1,216 @@
+/*
+ * itg.h — C ABI of libitg, an MI355X (gfx950) backend for the dense tensor
+ * storage of ITensors.jl.
+ *
+ * Conventions shared by every entry point
+ *  - Tensors are flat device buffers, column-major over their index order
+ *    (first index fastest), exactly the layout of NDTensors `Dense` storage
+ *    (src/tensor/cudense.jl:19 reshapes `data` to `dims(inds)...`), so a Julia
+ *    shim passes `data` pointers with no copy.
+ *  - dtype: ITG_F64 or ITG_F32 (storage type of every tensor argument).
+ *  - mode : arithmetic of a contraction. ITG_MODE_NATIVE computes in the
+ *    storage type (FP64 MFMA for F64); ITG_MODE_F32 rounds operands to FP32
+ *    and uses FP32 MFMA; ITG_MODE_BF16X3 splits each operand into hi+lo BF16
+ *    parts and sums the three leading products on BF16 MFMA with FP32
+ *    accumulation. Results are written in the storage type.
+ *  - Every call returns 0 (ITG_OK) or a negative ITG_E* code; the message is
+ *    available from itg_last_error(ctx).
+ *  - One context owns one HIP stream; calls are asynchronous on it except
+ *    those returning host scalars (dot, nrm2, truncate, eigh/svd `k`), which
+ *    synchronise, as the reference does at cudense.jl:25,27 and
+ *    cutruncate.jl:10,38,56,82. A context is used by one host thread.
+ *  - Outputs whose size is data dependent (truncated factorisations) are
+ *    written into caller-allocated buffers of the untruncated size; the
+ *    library reports the kept dimension k and the caller uses the leading k
+ *    columns (column-major, so no compaction is needed).
+ */
+#ifndef ITG_H
+#define ITG_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct itg_ctx itg_ctx;
+
+enum itg_status {
+  ITG_OK = 0,
+  ITG_EDIM = -1,         /* shape/label mismatch  (Julia: DimensionMismatch) */
+  ITG_EDTYPE = -2,       /* unsupported dtype     (Julia: ArgumentError)     */
+  ITG_EOOM = -3,         /* device allocation failed                         */
+  ITG_EHIP = -4,         /* HIP runtime error                                */
+  ITG_EUNSUPPORTED = -5, /* valid request this build does not implement      */
+  ITG_EARG = -6,         /* invalid argument      (Julia: ArgumentError)     */
+  ITG_ECONV = -7         /* iterative method did not converge                */
+};
+
+enum itg_dtype { ITG_F64 = 0, ITG_F32 = 1 };
+enum itg_mode { ITG_MODE_NATIVE = 0, ITG_MODE_F32 = 1, ITG_MODE_BF16X3 = 2 };
+
+/* truncate semantics: CPU NDTensors (default) or the reference GPU kernel
+ * (src/tensor/cutruncate.jl:1-93), kept for parity with its golden vectors. */
+enum itg_trunc_semantics { ITG_TRUNC_NDTENSORS = 0, ITG_TRUNC_REFGPU = 1 };
+
+/* ---------------------------------------------------------------- context */
+/* Replaces CUDA.jl device/stream setup (src/ITensorsGPU.jl:3-6). */
+int itg_init(int device, itg_ctx** ctx);
+int itg_destroy(itg_ctx* ctx);
+int itg_sync(itg_ctx* ctx);
+const char* itg_last_error(itg_ctx* ctx);
+/* Opaque hipStream_t of the context (for callers that record events). */
+void* itg_stream(itg_ctx* ctx);
+/* Device properties: number of CUs and the gfx arch number (e.g. 950). */
+int itg_device_info(itg_ctx* ctx, int* num_cu, int* gfx_arch, double* hbm_gb);
+
+/* ----------------------------------------------------------------- memory */
+/* Replace CuVector allocation / CuArray(x) / collect(x)
+ * (src/tensor/cudense.jl:4-12,62-72; src/cuitensor.jl:18-33). */
+int itg_malloc(itg_ctx* ctx, size_t bytes, void** ptr);
+int itg_free(itg_ctx* ctx, void* ptr);
+int itg_memcpy_h2d(itg_ctx* ctx, void* dst, const void* src, size_t bytes);
+int itg_memcpy_d2h(itg_ctx* ctx, void* dst, const void* src, size_t bytes);
+int itg_memcpy_d2d(itg_ctx* ctx, void* dst, const void* src, size_t bytes);
+/* x[i] = value, i < n (cudense.jl:6-10 `fill!`). */
+int itg_fill(itg_ctx* ctx, int dtype, int64_t n, double value, void* x);
+/* Cast between storage types (same length). */
+int itg_convert(itg_ctx* ctx, int64_t n, int src_dtype, const void* src,
+                int dst_dtype, void* dst);
+
+/* -------------------------------------------------------------------- rng */
+/* Gaussian fill replacing CURAND randn! (src/cuitensor.jl:37,43-44;
+ * src/mps/cumps.jl:47): Philox4x32-10 (Salmon et al., SC'11) keyed by `seed`,
+ * counter = offset/4 + i/4, Box-Muller on each 32-bit pair. Element i of the
+ * stream is the same on the host oracle and on the device. */
+int itg_randn(itg_ctx* ctx, int dtype, int64_t n, uint64_t seed,
+              uint64_t offset, void* x);
+
+/* ------------------------------------------------------- data movement */
+/* B = alpha * permutedims(A, perm): B has rank `rank`, dims dimsA[perm[j]],
+ * and B[j0..] = A[...] with B's mode j being A's mode perm[j] (0-based).
+ * Bit-exact for alpha == 1. Replaces Base.permute!/permutedims!!
+ * (src/tensor/cudense.jl:38-60,447-500). */
+int itg_permute(itg_ctx* ctx, int dtype, int rank, const int64_t* dimsA,
+                const int32_t* perm, double alpha, const void* A, void* B);
+/* B = alpha * permutedims(A, perm) + beta * B, B dims = dimsA[perm].
+ * Replaces the permuted +/- (src/tensor/cudense.jl:333-445). */
+int itg_add_permuted(itg_ctx* ctx, int dtype, int rank, const int64_t* dimsA,
+                     const int32_t* perm, double alpha, const void* A,
+                     double beta, void* B);
+
+/* ------------------------------------------------------------------ BLAS-1 */
+/* x *= alpha (cudense.jl:22,502). */
+int itg_scal(itg_ctx* ctx, int dtype, int64_t n, double alpha, void* x);
+/* y = alpha*x + beta*y. */
+int itg_axpby(itg_ctx* ctx, int dtype, int64_t n, double alpha, const void* x,
+              double beta, void* y);
+/* FP64-accumulated dot and 2-norm returned to the host (cudense.jl:25-27). */
+int itg_dot(itg_ctx* ctx, int dtype, int64_t n, const void* x, const void* y,
+            double* out);
+int itg_nrm2(itg_ctx* ctx, int dtype, int64_t n, const void* x, double* out);
+
+/* ------------------------------------------------------------ contraction */
+/* C = alpha * sum_{contracted} A * B + beta * C.
+ * Labels follow the einsum convention: a label present in A and B and absent
+ * from C is summed over; every label of C appears in exactly one of A or B.
+ * Ranks 0..ITG_MAX_RANK. Replaces contract!!/_contract!/_gemm_contract!/
+ * outer! (src/tensor/cudense.jl:74-110,129-331; src/tensor/dense.jl:1-48):
+ * index permutations are fused into the operand loads and the epilogue of one
+ * MFMA kernel. */
+#define ITG_MAX_RANK 16
+int itg_contract(itg_ctx* ctx, int dtype, int mode,
+                 int rA, const int64_t* dimsA, const int32_t* labelsA, const void* A,
+                 int rB, const int64_t* dimsB, const int32_t* labelsB, const void* B,
+                 int rC, const int64_t* dimsC, const int32_t* labelsC, void* C,
+                 double alpha, double beta);
+/* Plain column-major GEMM C(m,n) = alpha*op(A)*op(B) + beta*C, op = 'N'|'T'
+ * (order-2 `*`, src/tensor/culinearalgebra.jl:9-17). */
+int itg_gemm(itg_ctx* ctx, int dtype, int mode, char transA, char transB,
+             int64_t m, int64_t n, int64_t k, double alpha, const void* A,
+             int64_t lda, const void* B, int64_t ldb, double beta, void* C,
+             int64_t ldc);
+
+/* ------------------------------------------------------------- truncation */
+/* P: host array of n non-increasing weights (squared singular values or
+ * density-matrix eigenvalues). Returns the kept count, the truncation error
+ * and `docut` exactly as truncate! does (src/tensor/cutruncate.jl:1-93 for
+ * ITG_TRUNC_REFGPU; NDTensors' CPU truncate! for ITG_TRUNC_NDTENSORS). */
+int itg_truncate(itg_ctx* ctx, const double* P, int64_t n, int64_t maxdim,
+                 int64_t mindim, double cutoff, int absolute_cutoff,
+                 int relative_cutoff, int semantics, int64_t* k,
+                 double* truncerr, double* docut);
+/* Same decision for weights already resident on the device (length n),
+ * computed by one kernel; one 3-value read-back. */
+int itg_truncate_device(itg_ctx* ctx, const double* P_dev, int64_t n,
+                        int64_t maxdim, int64_t mindim, double cutoff,
+                        int absolute_cutoff, int relative_cutoff,
+                        int semantics, int64_t* k, double* truncerr,
+                        double* docut);
+
+/* -------------------------------------------------------- factorisations */
+/* Symmetric eigendecomposition A = U diag(w) U^T of a column-major n x n
+ * matrix (lower triangle read). Eigenvalues are returned in DESCENDING order
+ * into `w` (device, FP64, length n) and the matching eigenvectors into the
+ * columns of U (device, n x n, dtype). Then truncate! decides k
+ * (src/tensor/culinearalgebra.jl:74-108: syevd, reverse, truncate!; mindim
+ * is not applied, as there). A is destroyed. Pass maxdim <= 0 for "no
+ * maxdim" and cutoff < 0 for no truncation (k = n). */
+int itg_eigh_trunc(itg_ctx* ctx, int dtype, int64_t n, void* A, double* w,
+                   void* U, int64_t maxdim, double cutoff, int absolute_cutoff,
+                   int relative_cutoff, int64_t* k, double* truncerr,
+                   double* docut);
+/* Truncated SVD of a column-major m x n matrix A = U diag(S) V^T
+ * (src/tensor/culinearalgebra.jl:33-72). r = min(m,n). U: m x r, S: r
+ * (FP64, descending), V: n x r (device). P = S.^2 feeds truncate! with
+ * maxdim/mindim/cutoff; k is the kept count. A is destroyed. */
+int itg_svd_trunc(itg_ctx* ctx, int dtype, int64_t m, int64_t n, void* A,
+                  void* U, double* S, void* V, int64_t maxdim, int64_t mindim,
+                  double cutoff, int absolute_cutoff, int relative_cutoff,
+                  int64_t* k, double* truncerr, double* docut);
+/* Thin QR of a column-major m x n matrix: Q m x r, R r x n, r = min(m,n)
+ * (src/tensor/culinearalgebra.jl:110-121). A is not modified. */
+int itg_qr(itg_ctx* ctx, int dtype, int64_t m, int64_t n, const void* A,
+           void* Q, void* R);
+
+/* ------------------------------------------- effective Hamiltonian (2-site) */
+/* out(a',s1',s2',b') = sum L(a',wl,a) W1(wl,wm,s1',s1) W2(wm,wr,s2',s2)
+ *                          R(b',wr,b) psi(a,s1,s2,b)
+ * The ProjMPO product of ITensors (reached through dmrg, test/dmrg.jl:27):
+ * contraction order L.psi -> .W1 -> .W2 -> .R, with W1.W2 fused into one
+ * (w d d)x(w d d) operator applied in a single HBM pass. `work` must hold
+ * heff2_work_elems() elements of dtype (two intermediate tensors). */
+int64_t itg_heff2_work_elems(int64_t chiL, int64_t w, int64_t d, int64_t chiR);
+int itg_heff2_apply(itg_ctx* ctx, int dtype, int mode, int64_t chiL, int64_t w,
+                    int64_t d, int64_t chiR, const void* L, const void* W1,
+                    const void* W2, const void* R, const void* psi, void* out,
+                    void* work);
+/* Diagonal of the same operator, shape (chiL,d,d,chiR). */
+int itg_heff2_diag(itg_ctx* ctx, int dtype, int64_t chiL, int64_t w, int64_t d,
+                   int64_t chiR, const void* L, const void* W1, const void* W2,
+                   const void* R, void* diag);
+
+/* ------------------------------------ learned Davidson preconditioner (MLP) */
+/* No counterpart in the reference. A 3-layer MLP (in -> h -> h -> out, GELU)
+ * over feature rows; FP32 weights, row-major activations X[rows][in].
+ * Parameters are packed [W1(h x in), b1(h), W2(h x h), b2(h), W3(out x h),
+ * b3(out)]; `acts` holds 4*rows*h floats of saved activations for backward. */
+int64_t itg_mlp_param_count(int in, int hidden, int out);
+int itg_mlp_forward(itg_ctx* ctx, int rows, int in, int hidden, int out,
+                    const float* params, const float* X, float* Y, float* acts);
+/* Mean-squared-error loss against T; accumulates parameter gradients into
+ * grad (overwritten) and returns the loss to the host. */
+int itg_mlp_backward(itg_ctx* ctx, int rows, int in, int hidden, int out,
+                     const float* params, const float* X, const float* T,
+                     const float* acts, const float* Y, float* grad,
+                     double* loss);
+/* One Adam step (Kingma & Ba 2015) on packed parameters. */
+int itg_mlp_adam(itg_ctx* ctx, int64_t nparam, float* params, const float* grad,
+                 float* m, float* v, int step, float lr, float beta1,
+                 float beta2, float eps);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* ITG_H */

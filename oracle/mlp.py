@@ -1,0 +1,72 @@
+"""Forward/backward/Adam of the 3-layer GELU MLP used as a learned Davidson
+preconditioner (no reference counterpart; the packing matches itg.h).
+
+Y = W3 g(W2 g(W1 x + b1) + b2) + b3, g = exact GELU 0.5 x (1 + erf(x/sqrt2)),
+loss = mean over all rows*out entries of (Y - T)^2.
+"""
+import numpy as np
+from scipy.special import erf
+
+SQRT1_2 = 0.7071067811865476
+INV_SQRT_2PI = 0.3989422804014327
+
+
+def gelu(x):
+    return 0.5 * x * (1.0 + erf(x * SQRT1_2))
+
+
+def gelu_grad(x):
+    return 0.5 * (1.0 + erf(x * SQRT1_2)) + x * INV_SQRT_2PI * np.exp(-0.5 * x * x)
+
+
+def param_count(i, h, o):
+    return h * i + h + h * h + h + o * h + o
+
+
+def unpack(p, i, h, o):
+    p = np.asarray(p, dtype=np.float64)
+    sizes = [h * i, h, h * h, h, o * h, o]
+    out, s = [], 0
+    for n in sizes:
+        out.append(p[s:s + n])
+        s += n
+    W1, b1, W2, b2, W3, b3 = out
+    return W1.reshape(h, i), b1, W2.reshape(h, h), b2, W3.reshape(o, h), b3
+
+
+def forward(p, X, i, h, o):
+    W1, b1, W2, b2, W3, b3 = unpack(p, i, h, o)
+    X = np.asarray(X, dtype=np.float64)
+    Z1 = X @ W1.T + b1
+    A1 = gelu(Z1)
+    Z2 = A1 @ W2.T + b2
+    A2 = gelu(Z2)
+    Y = A2 @ W3.T + b3
+    return Y, (Z1, A1, Z2, A2)
+
+
+def loss_and_grad(p, X, T, i, h, o):
+    W1, b1, W2, b2, W3, b3 = unpack(p, i, h, o)
+    Y, (Z1, A1, Z2, A2) = forward(p, X, i, h, o)
+    R = Y.shape[0]
+    diff = Y - np.asarray(T, dtype=np.float64)
+    loss = float(np.mean(diff ** 2))
+    dY = 2.0 * diff / (R * o)
+    gW3 = dY.T @ A2
+    gb3 = dY.sum(0)
+    dZ2 = (dY @ W3) * gelu_grad(Z2)
+    gW2 = dZ2.T @ A1
+    gb2 = dZ2.sum(0)
+    dZ1 = (dZ2 @ W2) * gelu_grad(Z1)
+    gW1 = dZ1.T @ np.asarray(X, dtype=np.float64)
+    gb1 = dZ1.sum(0)
+    g = np.concatenate([gW1.ravel(), gb1, gW2.ravel(), gb2, gW3.ravel(), gb3])
+    return loss, g
+
+
+def adam(p, g, m, v, step, lr=1e-3, b1=0.9, b2=0.999, eps=1e-8):
+    m = b1 * m + (1 - b1) * g
+    v = b2 * v + (1 - b2) * g * g
+    mh = m / (1 - b1 ** step)
+    vh = v / (1 - b2 ** step)
+    return p - lr * mh / (np.sqrt(vh) + eps), m, v

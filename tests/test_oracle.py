@@ -1,0 +1,248 @@
+"""Pins the CPU oracle itself: golden vectors, known answers, brute force and
+exact diagonalisation. No GPU needed."""
+import itertools
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+
+from oracle import dmrg as odmrg
+from oracle import linalg as olinalg
+from oracle import mlp as omlp
+from oracle import models
+from oracle import philox
+from oracle import tensor_ops as ops
+from oracle.truncate import ndtensors_truncate, refgpu_truncate
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+
+
+# ------------------------------------------------------------- truncate!
+def test_truncate_reference_golden_vectors_refgpu():
+    """All three vectors of test/test_cutruncate.jl:9-17 under the reference
+    GPU semantics."""
+    cases = json.load(open(os.path.join(GOLD, "truncate_ref_vectors.json")))["cases"]
+    for c in cases:
+        n, terr, docut = refgpu_truncate(c["P"], **c["kwargs"])
+        kept = (list(c["P"][:n]) if max(c["P"]) > 0 else [0.0])
+        assert kept == c["kept"]
+        assert math.isclose(terr, c["truncerr"], rel_tol=1e-12, abs_tol=1e-15)
+        assert math.isclose(docut, c["docut"], rel_tol=1e-12, abs_tol=1e-15)
+
+
+def test_truncate_ndtensors_semantics_on_golden_vectors():
+    """CPU NDTensors semantics reproduce vectors 1 and 2; vector 3 is the
+    documented divergence (per-weight vs cumulative discard): n=2, docut .25."""
+    cases = json.load(open(os.path.join(GOLD, "truncate_ref_vectors.json")))["cases"]
+    n, terr, docut = ndtensors_truncate(cases[0]["P"], **cases[0]["kwargs"])
+    assert (n, terr, docut) == (1, 0.0, 0.0)
+    n, terr, docut = ndtensors_truncate(cases[1]["P"], **cases[1]["kwargs"])
+    assert n == 2 and math.isclose(terr, 0.15) and math.isclose(docut, 0.3)
+    n, terr, docut = ndtensors_truncate(cases[2]["P"], **cases[2]["kwargs"])
+    assert n == 2 and math.isclose(terr, 0.1) and math.isclose(docut, 0.25)
+
+
+def test_truncate_properties():
+    rng = np.random.default_rng(3)
+    for trial in range(200):
+        m = int(rng.integers(1, 40))
+        P = np.sort(rng.random(m) ** 6)[::-1]
+        cutoff = 10.0 ** rng.uniform(-8, -1)
+        maxdim = int(rng.integers(1, m + 1))
+        n, terr, _ = ndtensors_truncate(P, maxdim=maxdim, cutoff=cutoff)
+        assert 1 <= n <= maxdim
+        disc = P[n:].sum() / P.sum()
+        assert math.isclose(disc, terr, rel_tol=1e-10, abs_tol=1e-300)
+        if n < maxdim:   # stopped by the cutoff: one more would exceed it
+            assert disc <= cutoff * (1 + 1e-12)
+            assert disc + P[n - 1] / P.sum() > cutoff or n == 1
+
+
+# ------------------------------------------------------------- contraction
+def test_contract_matches_bruteforce_small():
+    rng = np.random.default_rng(0)
+    dims = {"i": 2, "j": 3, "k": 4, "l": 5, "a": 3}
+    for la, lb in [("ijk", "jkl"), ("ikj", "lkj"), ("ijkl", "kla"), ("ijkl", "jkl"),
+                   ("i", "j"), ("ij", "ij"), ("", "ij"), ("jki", "")]:
+        A = rng.standard_normal([dims[x] for x in la])
+        B = rng.standard_normal([dims[x] for x in lb])
+        lc = ops.itensor_output_labels(list(la), list(lb))
+        R1 = ops.contract(A, list(la), B, list(lb), lc)
+        R2 = ops.contract_bruteforce(A, list(la), B, list(lb), lc, dims)
+        np.testing.assert_allclose(R1, R2, rtol=1e-13, atol=1e-13)
+
+
+def test_mode_numbers_follow_reference_order():
+    # contracted (A's order) -> A-only -> B-only, cudense.jl:255-283
+    a, b, c = ops.mode_numbers(["i", "j", "k"], ["k", "l", "j"], ["i", "l"])
+    assert a == [3, 1, 2] and b == [2, 4, 1] and c == [3, 4]
+
+
+def test_permute_closed_form_transpose():
+    """test/test_cudense.jl:34-42: A[i,j] = i*j permuted to (j,i) equals
+    vec(transpose(A))."""
+    A = np.array([[float((ii + 1) * (jj + 1)) for jj in range(3)] for ii in range(2)])
+    B = ops.permute(A, [1, 0])
+    assert np.array_equal(ops.to_flat(B), ops.to_flat(A.T))
+
+
+def test_add_closed_form():
+    A = np.ones((2, 3))
+    B = 2.0 * np.ones((3, 2))
+    C = ops.add_permuted(A, [1, 0], 1.0, B, 1.0)
+    assert np.array_equal(C, 3.0 * np.ones((3, 2)))
+
+
+# ------------------------------------------------------------------ philox
+def test_philox_known_answers():
+    """Random123 kat_vectors, philox4x32 10 rounds."""
+    kat = [
+        ((0, 0, 0, 0), (0, 0), (0x6627e8d5, 0xe169c58d, 0xbc57ac4c, 0x9b00dbd8)),
+        ((0xffffffff,) * 4, (0xffffffff,) * 2, (0x408f276d, 0x41c83b0e, 0xa20bc7c6, 0x6d5451fd)),
+        ((0x243f6a88, 0x85a308d3, 0x13198a2e, 0x03707344), (0xa4093822, 0x299f31d0),
+         (0xd16cfe09, 0x94fdcceb, 0x5001e420, 0x24126ea1)),
+    ]
+    for ctr, key, want in kat:
+        got = philox.philox4x32_10([np.array([c], dtype=np.uint64) for c in ctr], key)
+        assert tuple(int(g[0]) for g in got) == want
+
+
+def test_randn_moments_and_offsets():
+    x = philox.randn(200000, seed=7)
+    assert abs(x.mean()) < 0.01 and abs(x.std() - 1) < 0.01
+    y = philox.randn(10, seed=7, offset=1001)
+    assert np.array_equal(y, x[1001:1011])
+
+
+# ------------------------------------------------------------------ linalg
+def test_linalg_invariants():
+    rng = np.random.default_rng(1)
+    A = rng.standard_normal((12, 9))
+    U, S, V, terr, _ = olinalg.svd_trunc(A)
+    np.testing.assert_allclose(U @ np.diag(S) @ V.T, A, atol=1e-13)
+    np.testing.assert_allclose(U.T @ U, np.eye(9), atol=1e-14)
+    Q, R = olinalg.qr(A)
+    np.testing.assert_allclose(Q @ R, A, atol=1e-13)
+    rho = A @ A.T
+    w, U, terr, _ = olinalg.eigh_trunc(rho, maxdim=5, cutoff=0.0)
+    assert np.all(np.diff(w) <= 0) and U.shape[1] == 5
+    full = np.sort(np.linalg.eigvalsh(rho))[::-1]
+    assert math.isclose(terr, full[5:].sum() / full.sum(), rel_tol=1e-10)
+
+
+# ------------------------------------------------------------------- models
+@pytest.mark.parametrize("S,N", [(0.5, 6), (1.0, 4)])
+def test_heisenberg_mpo_equals_ed(S, N):
+    H = models.mpo_to_matrix(models.heisenberg_mpo(N, S=S))
+    Hed = models.bond_hamiltonian(N, [(j, j + 1) for j in range(N - 1)], S=S).toarray()
+    np.testing.assert_allclose(H, Hed, atol=1e-14)
+
+
+def test_tfim_mpo_equals_ed():
+    H = models.mpo_to_matrix(models.tfim_mpo(6))
+    np.testing.assert_allclose(H, models.tfim_hamiltonian(6).toarray(), atol=1e-14)
+
+
+def test_ed_energies_known():
+    chain = lambda n: [(j, j + 1) for j in range(n - 1)]
+    assert math.isclose(models.ground_energy(models.bond_hamiltonian(10, chain(10))),
+                        -4.258035207282881, rel_tol=1e-12)
+    assert math.isclose(models.ground_energy(models.bond_hamiltonian(12, chain(12))),
+                        -5.14209063284054, rel_tol=1e-12)
+    # TFIM closed form (test/dmrg.jl:79) against ED at N=10
+    assert math.isclose(models.ground_energy(models.tfim_hamiltonian(10)),
+                        models.tfim_exact_energy(10), rel_tol=1e-11)
+
+
+def test_cylinder_bond_count():
+    b = models.cylinder_bonds(16, 4)
+    assert len(b) == 16 * 4 + 15 * 4   # y-rings + x-bonds
+
+
+# -------------------------------------------------------------------- dmrg
+def test_projmpo_apply_equals_dense_operator():
+    rng = np.random.default_rng(5)
+    H = models.heisenberg_mpo(4)
+    psi = odmrg.random_mps(4, 2, 4, seed=3)
+    L = np.ones((1, 1, 1))
+    R = odmrg.update_right(np.ones((1, 1, 1)), psi[3], H[3])
+    x = rng.standard_normal((1, 2, 2, psi[2].shape[0]))
+    y = odmrg.heff_apply(L, H[0], H[1], odmrg.update_right(R, psi[2], H[2]), x)
+    # brute: dense effective operator
+    Rb = odmrg.update_right(R, psi[2], H[2])
+    n = x.size
+    Hm = np.zeros((n, n))
+    for c in range(n):
+        e = np.zeros(n)
+        e[c] = 1
+        Hm[:, c] = odmrg.heff_apply(L, H[0], H[1], Rb, e.reshape(x.shape)).ravel()
+    np.testing.assert_allclose(Hm, Hm.T, atol=1e-13)
+    np.testing.assert_allclose(y.ravel(), Hm @ x.ravel(), atol=1e-13)
+    dg = odmrg.heff_diag(L, H[0], H[1], Rb).ravel()
+    np.testing.assert_allclose(dg, np.diag(Hm), atol=1e-13)
+
+
+def test_random_mps_is_right_canonical():
+    psi = odmrg.random_mps(8, 2, 6, seed=0)
+    for A in psi[1:]:
+        assert odmrg.right_orthonormal_error(A) < 1e-13
+    assert math.isclose(np.linalg.norm(psi[0]), 1.0, rel_tol=1e-13)
+
+
+def test_dmrg_heisenberg_n12_matches_ed():
+    H = models.heisenberg_mpo(12)
+    psi = odmrg.random_mps(12, 2, 10, seed=0)
+    E = odmrg.dmrg(H, psi, maxdims=[10, 20, 64, 64, 64, 64], cutoffs=[1e-12] * 6)
+    assert abs(E - (-5.14209063284054)) < 1e-10
+    assert abs(odmrg.mps_energy(H, psi) - E) < 1e-10
+
+
+def test_dmrg_config1_heisenberg_n20():
+    """S=1/2 Heisenberg N=20, random MPS linkdims 10 seed 0, 5 sweeps maxdim
+    [10,20,100,100,200], cutoff 1e-10, 2 Krylov vectors: within 1e-7 of the
+    ED ground state -8.682473334398939 and above it (variational)."""
+    H = models.heisenberg_mpo(20)
+    psi = odmrg.random_mps(20, 2, 10, seed=0)
+    E = odmrg.dmrg(H, psi, maxdims=[10, 20, 100, 100, 200], cutoffs=[1e-10] * 5,
+                   krylovdim=2)
+    E0 = -8.682473334398939
+    assert E0 - 1e-10 <= E < E0 + 1e-7
+
+
+def test_dmrg_tfim_reference_test():
+    """test/dmrg.jl:58-81 at N=32: relative error < 1e-2 (we reach ~1e-6)."""
+    N = 32
+    H = models.tfim_mpo(N)
+    psi = odmrg.random_mps(N, 2, 10, seed=432)
+    E = odmrg.dmrg(H, psi, maxdims=[10, 20, 20, 20, 20], cutoffs=[1e-12] * 5)
+    Ex = models.tfim_exact_energy(N)
+    assert abs((E - Ex) / Ex) < 1e-2
+    assert E >= Ex - 1e-10   # variational
+
+
+def test_davidson_eigen_residual():
+    """test/test_cuiterativesolvers.jl:13-27: M(v) ~ lambda v on 10x10."""
+    rng = np.random.default_rng(2)
+    A = rng.standard_normal((10, 10))
+    M = A @ A.T
+    lam, v, _ = odmrg.davidson(lambda x: M @ x, rng.standard_normal(10), maxiter=10)
+    np.testing.assert_allclose(M @ v, lam * v, atol=1e-9)
+    assert math.isclose(lam, np.linalg.eigvalsh(M)[0], rel_tol=1e-9, abs_tol=1e-9)
+
+
+# --------------------------------------------------------------------- mlp
+def test_mlp_gradient_finite_differences():
+    rng = np.random.default_rng(4)
+    i, h, o, rows = 5, 7, 3, 6
+    p = rng.standard_normal(omlp.param_count(i, h, o)) * 0.5
+    X = rng.standard_normal((rows, i))
+    T = rng.standard_normal((rows, o))
+    loss, g = omlp.loss_and_grad(p, X, T, i, h, o)
+    eps = 1e-6
+    for idx in rng.choice(p.size, 25, replace=False):
+        pp = p.copy(); pp[idx] += eps
+        pm = p.copy(); pm[idx] -= eps
+        fd = (omlp.loss_and_grad(pp, X, T, i, h, o)[0] - omlp.loss_and_grad(pm, X, T, i, h, o)[0]) / (2 * eps)
+        assert abs(fd - g[idx]) < 1e-7 * max(1.0, abs(fd))
