@@ -62,6 +62,10 @@ int itg_sync(itg_ctx* ctx);
 const char* itg_last_error(itg_ctx* ctx);
 /* Opaque hipStream_t of the context (for callers that record events). */
 void* itg_stream(itg_ctx* ctx);
+/* Run the context's work on a caller-owned hipStream_t (e.g. the stream of
+ * the host framework that owns the device buffers); NULL restores the
+ * context's own stream. The caller keeps the stream alive. */
+int itg_set_stream(itg_ctx* ctx, void* stream);
 /* Device properties: number of CUs and the gfx arch number (e.g. 950). */
 int itg_device_info(itg_ctx* ctx, int* num_cu, int* gfx_arch, double* hbm_gb);
 

@@ -1,0 +1,88 @@
+// Internal declarations shared by the libitg translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/itg.h"
+
+struct itg_plan;  // contraction plan (gett.hip)
+
+struct itg_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;       // stream all work is queued on
+  hipStream_t own_stream = nullptr;   // the context's own stream
+  std::string err;
+  int num_cu = 0;
+  int gfx = 0;
+  double hbm_gb = 0.0;
+  // grow-only device workspace for internal temporaries
+  void* ws = nullptr;
+  size_t ws_bytes = 0;
+  // small device scratch for reductions + pinned host mirror
+  double* red = nullptr;        // device, kRedSlots doubles
+  double* red_host = nullptr;   // pinned host, kRedSlots doubles
+  // contraction plan cache (keyed by the canonical contraction signature)
+  std::unordered_map<std::string, itg_plan*> plans;
+  size_t plan_bytes = 0;
+};
+
+constexpr int kRedSlots = 16384;
+
+// ---------------------------------------------------------------- errors
+int itg_fail(itg_ctx* ctx, int code, const char* fmt, ...);
+
+#define ITG_HIP(ctx, call)                                                    \
+  do {                                                                        \
+    hipError_t e_ = (call);                                                   \
+    if (e_ != hipSuccess)                                                     \
+      return itg_fail((ctx), ITG_EHIP, "%s failed: %s (%s:%d)", #call,        \
+                      hipGetErrorString(e_), __FILE__, __LINE__);             \
+  } while (0)
+
+#define ITG_LAUNCH_CHECK(ctx)                                                 \
+  do {                                                                        \
+    hipError_t e_ = hipGetLastError();                                        \
+    if (e_ != hipSuccess)                                                     \
+      return itg_fail((ctx), ITG_EHIP, "kernel launch failed: %s (%s:%d)",    \
+                      hipGetErrorString(e_), __FILE__, __LINE__);             \
+  } while (0)
+
+#define ITG_CHECK_CTX(ctx)                                                    \
+  do {                                                                        \
+    if (!(ctx)) return ITG_EARG;                                              \
+  } while (0)
+
+#define ITG_CHECK_DTYPE(ctx, dt)                                              \
+  do {                                                                        \
+    if ((dt) != ITG_F64 && (dt) != ITG_F32)                                   \
+      return itg_fail((ctx), ITG_EDTYPE, "unsupported dtype %d", (int)(dt));  \
+  } while (0)
+
+static inline size_t itg_dtype_size(int dt) { return dt == ITG_F64 ? 8 : 4; }
+
+// Grow-only workspace; the returned pointer is valid until the next call.
+int itg_workspace(itg_ctx* ctx, size_t bytes, void** out);
+
+// Reduction helper: sums `nparts` doubles already in ctx->red[0..nparts) on
+// device and copies the result (and extra slots) to the host synchronously.
+int itg_read_reduction(itg_ctx* ctx, int nslots, double* host_out);
+
+static inline unsigned itg_blocks(int64_t n, int threads, int64_t cap = 65535 * 8) {
+  int64_t b = (n + threads - 1) / threads;
+  if (b < 1) b = 1;
+  if (b > cap) b = cap;
+  return (unsigned)b;
+}
+
+// Internal (non-exported) entry points used across translation units.
+int itg_gemm_internal(itg_ctx* ctx, int dtype, int mode, char ta, char tb,
+                      int64_t m, int64_t n, int64_t k, double alpha,
+                      const void* A, int64_t lda, const void* B, int64_t ldb,
+                      double beta, void* C, int64_t ldc);
+void itg_plan_cache_clear(itg_ctx* ctx);

@@ -1,0 +1,254 @@
+"""Storage-level operations on device buffers, one call into libitg each.
+
+A device tensor is a flat torch tensor on the GPU holding column-major data
+(first index fastest), the layout of NDTensors `Dense` storage
+(src/tensor/cudense.jl:19). Shapes travel separately as tuples of ints.
+These functions are what a Julia `RocDense` shim would `ccall`; the ITensor
+layer in `itensor.py` builds on them.
+"""
+import ctypes as C
+import math
+
+import numpy as np
+
+from . import _lib
+from ._lib import ctx, i32arr, i64arr, lib, ptr, dtype_code
+
+
+def _torch():
+    import torch
+    return torch
+
+
+def empty(n, dtype="float64"):
+    torch = _torch()
+    return torch.empty(max(int(n), 0), dtype=getattr(torch, dtype), device="cuda")
+
+
+def zeros(n, dtype="float64"):
+    torch = _torch()
+    return torch.zeros(max(int(n), 0), dtype=getattr(torch, dtype), device="cuda")
+
+
+def prod(dims):
+    p = 1
+    for d in dims:
+        p *= int(d)
+    return p
+
+
+def to_device(host_flat, dtype="float64"):
+    """roc(.) data movement: host column-major flat array -> device buffer."""
+    torch = _torch()
+    a = np.ascontiguousarray(np.asarray(host_flat, dtype=np.dtype(dtype)).ravel())
+    t = empty(a.size, dtype)
+    if a.size:
+        c = ctx()
+        c.check(lib().itg_memcpy_h2d(c.ptr, ptr(t), a.ctypes.data_as(C.c_void_p), a.nbytes))
+    return t
+
+
+def to_host(t):
+    """cpu(.) data movement: device buffer -> host numpy array (flat)."""
+    torch = _torch()
+    out = np.empty(t.numel(), dtype=np.float64 if t.dtype == torch.float64 else np.float32)
+    if out.size:
+        c = ctx()
+        c.check(lib().itg_memcpy_d2h(c.ptr, out.ctypes.data_as(C.c_void_p), ptr(t), out.nbytes))
+    return out
+
+
+def fill(t, value):
+    c = ctx()
+    c.check(lib().itg_fill(c.ptr, dtype_code(t), t.numel(), float(value), ptr(t)))
+    return t
+
+
+def randn(n, seed, offset=0, dtype="float64"):
+    t = empty(n, dtype)
+    c = ctx()
+    c.check(lib().itg_randn(c.ptr, dtype_code(t), int(n), int(seed), int(offset), ptr(t)))
+    return t
+
+
+def convert(t, dtype):
+    out = empty(t.numel(), dtype)
+    c = ctx()
+    c.check(lib().itg_convert(c.ptr, t.numel(), dtype_code(t), ptr(t), dtype_code(out), ptr(out)))
+    return out
+
+
+# ------------------------------------------------------------------ BLAS-1
+def scal(t, alpha):
+    c = ctx()
+    c.check(lib().itg_scal(c.ptr, dtype_code(t), t.numel(), float(alpha), ptr(t)))
+    return t
+
+
+def axpby(alpha, x, beta, y):
+    """y = alpha*x + beta*y (in place on y)."""
+    c = ctx()
+    c.check(lib().itg_axpby(c.ptr, dtype_code(y), y.numel(), float(alpha), ptr(x), float(beta),
+                            ptr(y)))
+    return y
+
+
+def dot(x, y):
+    c = ctx()
+    out = C.c_double()
+    c.check(lib().itg_dot(c.ptr, dtype_code(x), x.numel(), ptr(x), ptr(y), C.byref(out)))
+    return out.value
+
+
+def nrm2(x):
+    c = ctx()
+    out = C.c_double()
+    c.check(lib().itg_nrm2(c.ptr, dtype_code(x), x.numel(), ptr(x), C.byref(out)))
+    return out.value
+
+
+# ----------------------------------------------------------- data movement
+def permute(A, dimsA, perm, alpha=1.0, out=None):
+    """B = alpha*permutedims(A, perm) (0-based perm); returns (B, dimsB)."""
+    dimsB = tuple(int(dimsA[p]) for p in perm)
+    B = out if out is not None else empty(prod(dimsB), str(A.dtype).split(".")[-1])
+    c = ctx()
+    c.check(lib().itg_permute(c.ptr, dtype_code(A), len(dimsA), i64arr(dimsA), i32arr(perm),
+                              float(alpha), ptr(A), ptr(B)))
+    return B, dimsB
+
+
+def add_permuted(A, dimsA, perm, alpha, B, beta):
+    """B = alpha*permutedims(A, perm) + beta*B, in place on B."""
+    c = ctx()
+    c.check(lib().itg_add_permuted(c.ptr, dtype_code(A), len(dimsA), i64arr(dimsA), i32arr(perm),
+                                   float(alpha), ptr(A), float(beta), ptr(B)))
+    return B
+
+
+# ------------------------------------------------------------- contraction
+def contract(A, dimsA, labelsA, B, dimsB, labelsB, labelsC, mode="native", alpha=1.0,
+             beta=0.0, out=None):
+    """C = alpha*A.B + beta*C with einsum-style integer labels; returns (C, dimsC)."""
+    dmap = {}
+    for l, d in zip(labelsA, dimsA):
+        dmap[l] = int(d)
+    for l, d in zip(labelsB, dimsB):
+        dmap.setdefault(l, int(d))
+    try:
+        dimsC = tuple(dmap[l] for l in labelsC)
+    except KeyError as e:
+        raise _lib.DimensionMismatch("output label %r missing from inputs" % (e.args[0],))
+    if out is None:
+        out = empty(prod(dimsC), str(A.dtype).split(".")[-1])
+    c = ctx()
+    c.check(lib().itg_contract(c.ptr, dtype_code(A), _lib.MODES[mode],
+                               len(dimsA), i64arr(dimsA), i32arr(labelsA), ptr(A),
+                               len(dimsB), i64arr(dimsB), i32arr(labelsB), ptr(B),
+                               len(labelsC), i64arr(dimsC), i32arr(labelsC), ptr(out),
+                               float(alpha), float(beta)))
+    return out, dimsC
+
+
+def gemm(ta, tb, m, n, k, alpha, A, lda, B, ldb, beta, Cm, ldc, mode="native"):
+    c = ctx()
+    c.check(lib().itg_gemm(c.ptr, dtype_code(A), _lib.MODES[mode], ta.encode(), tb.encode(),
+                           m, n, k, float(alpha), ptr(A), lda, ptr(B), ldb, float(beta),
+                           ptr(Cm), ldc))
+    return Cm
+
+
+# -------------------------------------------------------------- truncation
+def truncate(P, maxdim=None, mindim=1, cutoff=0.0, absolute_cutoff=False,
+             relative_cutoff=True, semantics="ndtensors"):
+    """truncate!(P; ...) -> (n_kept, truncerr, docut) for host weights."""
+    a, pa = _lib.host_f64(P)
+    k, te, dc = C.c_int64(), C.c_double(), C.c_double()
+    c = ctx()
+    c.check(lib().itg_truncate(c.ptr, pa, a.size, int(maxdim or 0), int(mindim), float(cutoff),
+                               int(absolute_cutoff), int(relative_cutoff),
+                               _lib.TRUNC_REFGPU if semantics == "refgpu" else _lib.TRUNC_NDTENSORS,
+                               C.byref(k), C.byref(te), C.byref(dc)))
+    return k.value, te.value, dc.value
+
+
+def truncate_device(P, maxdim=None, mindim=1, cutoff=0.0, absolute_cutoff=False,
+                    relative_cutoff=True, semantics="ndtensors"):
+    k, te, dc = C.c_int64(), C.c_double(), C.c_double()
+    c = ctx()
+    c.check(lib().itg_truncate_device(c.ptr, ptr(P), P.numel(), int(maxdim or 0), int(mindim),
+                                      float(cutoff), int(absolute_cutoff), int(relative_cutoff),
+                                      _lib.TRUNC_REFGPU if semantics == "refgpu"
+                                      else _lib.TRUNC_NDTENSORS,
+                                      C.byref(k), C.byref(te), C.byref(dc)))
+    return k.value, te.value, dc.value
+
+
+# ----------------------------------------------------------- factorisations
+def eigh_trunc(A, n, maxdim=None, cutoff=-1.0, absolute_cutoff=False, relative_cutoff=True):
+    """Eigen-decomposition of a symmetric n x n device matrix (destroyed).
+    Returns (w_desc[:k] device, U device n x n buffer (first k columns), k,
+    truncerr, docut)."""
+    dt = str(A.dtype).split(".")[-1]
+    w = empty(n, "float64")
+    U = empty(n * n, dt)
+    k, te, dc = C.c_int64(), C.c_double(), C.c_double()
+    c = ctx()
+    c.check(lib().itg_eigh_trunc(c.ptr, dtype_code(A), n, ptr(A), ptr(w), ptr(U),
+                                 int(maxdim or 0), float(-1.0 if cutoff is None else cutoff),
+                                 int(absolute_cutoff), int(relative_cutoff),
+                                 C.byref(k), C.byref(te), C.byref(dc)))
+    return w, U, k.value, te.value, dc.value
+
+
+def svd_trunc(A, m, n, maxdim=None, mindim=1, cutoff=0.0, absolute_cutoff=False,
+              relative_cutoff=True):
+    dt = str(A.dtype).split(".")[-1]
+    r = min(m, n)
+    U = empty(m * r, dt)
+    S = empty(r, "float64")
+    V = empty(n * r, dt)
+    k, te, dc = C.c_int64(), C.c_double(), C.c_double()
+    c = ctx()
+    c.check(lib().itg_svd_trunc(c.ptr, dtype_code(A), m, n, ptr(A), ptr(U), ptr(S), ptr(V),
+                                int(maxdim or 0), int(mindim), float(cutoff),
+                                int(absolute_cutoff), int(relative_cutoff),
+                                C.byref(k), C.byref(te), C.byref(dc)))
+    return U, S, V, k.value, te.value, dc.value
+
+
+def qr(A, m, n):
+    dt = str(A.dtype).split(".")[-1]
+    r = min(m, n)
+    Q = empty(m * r, dt)
+    R = empty(r * n, dt)
+    c = ctx()
+    c.check(lib().itg_qr(c.ptr, dtype_code(A), m, n, ptr(A), ptr(Q), ptr(R)))
+    return Q, R
+
+
+# ------------------------------------------------------------------- Heff
+def heff2_work(chiL, w, d, chiR, dtype="float64"):
+    return empty(lib().itg_heff2_work_elems(chiL, w, d, chiR), dtype)
+
+
+def heff2_apply(L, W1, W2, R, psi, chiL, w, d, chiR, mode="native", out=None, work=None):
+    dt = str(psi.dtype).split(".")[-1]
+    if out is None:
+        out = empty(chiL * d * d * chiR, dt)
+    if work is None:
+        work = heff2_work(chiL, w, d, chiR, dt)
+    c = ctx()
+    c.check(lib().itg_heff2_apply(c.ptr, dtype_code(psi), _lib.MODES[mode], chiL, w, d, chiR,
+                                  ptr(L), ptr(W1), ptr(W2), ptr(R), ptr(psi), ptr(out),
+                                  ptr(work)))
+    return out
+
+
+def heff2_diag(L, W1, W2, R, chiL, w, d, chiR):
+    dt = str(L.dtype).split(".")[-1]
+    out = empty(chiL * d * d * chiR, dt)
+    c = ctx()
+    c.check(lib().itg_heff2_diag(c.ptr, dtype_code(L), chiL, w, d, chiR, ptr(L), ptr(W1),
+                                 ptr(W2), ptr(R), ptr(out)))
+    return out
