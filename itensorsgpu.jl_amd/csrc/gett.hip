@@ -269,7 +269,9 @@ __global__ __launch_bounds__(256, 2) void gett_kernel(GettArgs g) {
     }
   }
 
-  // epilogue: lane holds D[4*lg + r][lr] of each 16x16 block
+  // epilogue: lane holds D[row][lr] of each 16x16 block, with row = 4*lg + r
+  // for the FP32-accumulating MFMAs (4-row groups) and row = 4*r + lg for
+  // v_mfma_f64_16x16x4_f64 (1-row groups)
   if (g.partial) {
     CT* P = (CT*)g.partial + (int64_t)blockIdx.z * g.M * g.N;
 #pragma unroll
@@ -280,7 +282,7 @@ __global__ __launch_bounds__(256, 2) void gett_kernel(GettArgs g) {
         if (n >= g.N) continue;
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-          int64_t m = m0 + wm * 64 + i * 16 + 4 * lg + r;
+          int64_t m = m0 + wm * 64 + i * 16 + (TR::F64 ? 4 * r + lg : 4 * lg + r);
           if (m < g.M) P[n * g.M + m] = acc[i][j][r];
         }
       }
@@ -296,7 +298,7 @@ __global__ __launch_bounds__(256, 2) void gett_kernel(GettArgs g) {
       int64_t on = goff(g.tCn, g.sCn, n);
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        int64_t m = m0 + wm * 64 + i * 16 + 4 * lg + r;
+        int64_t m = m0 + wm * 64 + i * 16 + (TR::F64 ? 4 * r + lg : 4 * lg + r);
         if (m >= g.M) continue;
         int64_t o = on + goff(g.tCm, g.sCm, m);
         double v = g.alpha * (double)acc[i][j][r];
