@@ -181,19 +181,23 @@ int itg_qr(itg_ctx* ctx, int dtype, int64_t m, int64_t n, const void* A,
 /* ------------------------------------------- effective Hamiltonian (2-site) */
 /* out(a',s1',s2',b') = sum L(a',wl,a) W1(wl,wm,s1',s1) W2(wm,wr,s2',s2)
  *                          R(b',wr,b) psi(a,s1,s2,b)
- * The ProjMPO product of ITensors (reached through dmrg, test/dmrg.jl:27):
- * contraction order L.psi -> .W1 -> .W2 -> .R, with W1.W2 fused into one
- * (w d d)x(w d d) operator applied in a single HBM pass. `work` must hold
- * heff2_work_elems() elements of dtype (two intermediate tensors). */
-int64_t itg_heff2_work_elems(int64_t chiL, int64_t w, int64_t d, int64_t chiR);
-int itg_heff2_apply(itg_ctx* ctx, int dtype, int mode, int64_t chiL, int64_t w,
-                    int64_t d, int64_t chiR, const void* L, const void* W1,
-                    const void* W2, const void* R, const void* psi, void* out,
-                    void* work);
+ * with L (chiL,wl,chiL), W1 (wl,wm,d,d), W2 (wm,wr,d,d), R (chiR,wr,chiR),
+ * psi/out (chiL,d,d,chiR). The ProjMPO product of ITensors (reached through
+ * dmrg, test/dmrg.jl:27): contraction order L.psi -> .W1 -> .W2 -> .R, with
+ * W1.W2 fused into one (wl d d) x (d d wr) operator applied in a single HBM
+ * pass; the two large steps run on MFMA in `mode`, the W pass in FP64.
+ * `work` must hold itg_heff2_work_elems() elements of dtype. */
+int64_t itg_heff2_work_elems(int64_t chiL, int64_t wl, int64_t wm, int64_t wr,
+                             int64_t d, int64_t chiR);
+int itg_heff2_apply(itg_ctx* ctx, int dtype, int mode, int64_t chiL, int64_t wl,
+                    int64_t wm, int64_t wr, int64_t d, int64_t chiR,
+                    const void* L, const void* W1, const void* W2, const void* R,
+                    const void* psi, void* out, void* work);
 /* Diagonal of the same operator, shape (chiL,d,d,chiR). */
-int itg_heff2_diag(itg_ctx* ctx, int dtype, int64_t chiL, int64_t w, int64_t d,
-                   int64_t chiR, const void* L, const void* W1, const void* W2,
-                   const void* R, void* diag);
+int itg_heff2_diag(itg_ctx* ctx, int dtype, int64_t chiL, int64_t wl,
+                   int64_t wm, int64_t wr, int64_t d, int64_t chiR,
+                   const void* L, const void* W1, const void* W2, const void* R,
+                   void* diag);
 
 /* ------------------------------------ learned Davidson preconditioner (MLP) */
 /* No counterpart in the reference. A 3-layer MLP (in -> h -> h -> out, GELU)

@@ -1,0 +1,181 @@
+// Numerical core of the symmetric tridiagonal divide-and-conquer eigensolver
+// (Cuppen 1981; Gu & Eisenstat, SIAM J. Matrix Anal. Appl. 16 (1995)),
+// shared by the GPU kernels (eigh.hip) and a CPU harness used in tests.
+//
+// One merge solves diag(dl) + rho * w w^T (rho > 0, |w| = 1) after
+// deflation:
+//  - dc_deflate: drop entries with negligible w and rotate away one of each
+//    pair of (nearly) equal dl (the deflation rules of LAPACK dlaed2);
+//  - dc_secular_root: root i of f(x) = 1 + rho sum_j w_j^2/(dl_j - x) as
+//    (origin o, offset tau) with x = dl_o + tau, by a bracketed iteration on
+//    a two-pole rational model of f (fixed-weight scheme of Bunch, Nielsen &
+//    Sorensen 1978) with bisection as the safeguard;
+//  - dc_zhat: w recomputed from the computed roots (Gu-Eisenstat), which
+//    makes the eigenvectors zhat_i / (dl_i - x_j) orthogonal to working
+//    precision.
+#pragma once
+
+#include <math.h>
+#include <stdint.h>
+
+#ifdef __HIPCC__
+#define ITG_HD __host__ __device__
+#else
+#define ITG_HD
+#endif
+
+#define ITG_DC_EPS 1.1102230246251565e-16
+
+// f(tau) with origin o; also the derivative split into the part from poles on
+// the origin's side of the root (dpo) and from the far side (dpf).
+ITG_HD static inline double dc_secular_eval(int k, const double* dl, const double* w, double rho,
+                                            int o, double tau, int side, double* dpo,
+                                            double* dpf, double* absum) {
+  // side > 0: root right of the origin pole (origin side = j <= o)
+  // side < 0: root left of the origin pole  (origin side = j >= o)
+  double f = 1.0, po = 0.0, pf = 0.0, ab = 1.0;
+  double dlo = dl[o];
+  for (int j = 0; j < k; ++j) {
+    double del = (dl[j] - dlo) - tau;
+    double t = w[j] / del;
+    double term = rho * w[j] * t;
+    f += term;
+    ab += fabs(term);
+    double dt = rho * t * t;
+    bool origin_side = side > 0 ? (j <= o) : (j >= o);
+    if (origin_side) po += dt; else pf += dt;
+  }
+  *dpo = po;
+  *dpf = pf;
+  *absum = ab;
+  return f;
+}
+
+ITG_HD static inline void dc_secular_root(int i, int k, const double* dl, const double* w,
+                                          double rho, double w2sum, int* org_out,
+                                          double* tau_out) {
+  int o, side;
+  double lo, hi, D;
+  if (i == k - 1) {
+    o = i;
+    side = 1;
+    lo = 0.0;
+    hi = rho * w2sum;
+    D = 0.0;   // no pole on the right
+  } else {
+    double gap = dl[i + 1] - dl[i];
+    double mid = 0.5 * gap;
+    double f = 1.0;
+    for (int j = 0; j < k; ++j) f += rho * w[j] * w[j] / ((dl[j] - dl[i]) - mid);
+    if (f >= 0.0) {
+      o = i; side = 1; lo = 0.0; hi = mid; D = gap;
+    } else {
+      o = i + 1; side = -1; lo = mid - gap; hi = 0.0; D = -gap;
+    }
+  }
+  double tau = 0.5 * (lo + hi);
+  for (int it = 0; it < 200; ++it) {
+    double dpo, dpf, ab;
+    double f = dc_secular_eval(k, dl, w, rho, o, tau, side, &dpo, &dpf, &ab);
+    if (f > 0.0) hi = tau; else lo = tau;
+    if (fabs(f) <= 4.0 * ITG_DC_EPS * ab) break;
+    if (hi - lo <= 2.0 * ITG_DC_EPS * fmax(fabs(lo), fabs(hi))) break;
+    // two-pole model c + a/(-x) + b/(D - x) matching f and f' at tau
+    double a = tau * tau * dpo;
+    double xn;
+    if (D == 0.0 || dpf == 0.0) {
+      double c = f + a / tau;            // f - a/(-tau)
+      if (D != 0.0) {
+        // far side empty: c + a/(-x) = 0 still models f
+      }
+      xn = (c > 0.0) ? a / c : 0.5 * (lo + hi);
+    } else {
+      double b = (D - tau) * (D - tau) * dpf;
+      double c = f + a / tau - b / (D - tau);
+      double A2 = c, B2 = -(c * D + a + b), C2 = a * D;
+      double disc = B2 * B2 - 4.0 * A2 * C2;
+      if (disc < 0.0) disc = 0.0;
+      double sq = sqrt(disc);
+      double q = -0.5 * (B2 + (B2 >= 0.0 ? sq : -sq));
+      double x1 = (A2 != 0.0) ? q / A2 : (lo - 1.0);
+      double x2 = (q != 0.0) ? C2 / q : (lo - 1.0);
+      if (x1 > lo && x1 < hi) xn = x1;
+      else if (x2 > lo && x2 < hi) xn = x2;
+      else xn = 0.5 * (lo + hi);
+    }
+    if (!(xn > lo && xn < hi)) xn = 0.5 * (lo + hi);
+    if (fabs(xn - tau) <= 2.0 * ITG_DC_EPS * fabs(xn)) { tau = xn; break; }
+    tau = xn;
+  }
+  *org_out = o;
+  *tau_out = tau;
+}
+
+// zhat_i from the roots (x_j = dl[org_j] + tau_j):
+// zhat_i^2 = -prod_j (dl_i - x_j) / prod_{j != i} (dl_i - dl_j), up to 1/rho
+ITG_HD static inline double dc_zhat(int i, int k, const double* dl, const int* org,
+                                    const double* tau, const double* w) {
+  double p = (dl[i] - dl[org[i]]) - tau[i];
+  for (int j = 0; j < k; ++j) {
+    if (j == i) continue;
+    p *= ((dl[i] - dl[org[j]]) - tau[j]) / (dl[i] - dl[j]);
+  }
+  double z2 = -p;
+  if (z2 < 0.0) z2 = 0.0;
+  return copysign(sqrt(z2), w[i]);
+}
+
+// Deflation of one merge. Inputs sorted ascending by d (length s): d, z
+// (|z| = 1), col (column of Q holding each entry's vector); rho > 0.
+// Outputs: non-deflated (dl, w, gcol)[0..k); deflated (dval, dcol)[0..nd);
+// rotations (rcol pairs, rcs = (c, s)) to apply to Q's columns in order:
+// q_p <- c q_p + s q_n, q_n <- c q_n - s q_p. Returns k.
+ITG_HD static inline int dc_deflate(int s, double* d, double* z, const int* col, double rho,
+                                    double* dl, double* w, int* gcol, double* dval, int* dcol,
+                                    int* nd_out, int* rcol, double* rcs, int* nrot_out) {
+  double dmax = 0.0, zmax = 0.0;
+  for (int j = 0; j < s; ++j) {
+    dmax = fmax(dmax, fabs(d[j]));
+    zmax = fmax(zmax, fabs(z[j]));
+  }
+  double tol = 8.0 * ITG_DC_EPS * fmax(dmax, zmax);
+  int k = 0, nd = 0, nrot = 0, pj = -1;
+  if (rho * zmax <= tol) {
+    for (int j = 0; j < s; ++j) { dval[nd] = d[j]; dcol[nd] = col[j]; ++nd; }
+    *nd_out = nd;
+    *nrot_out = 0;
+    return 0;
+  }
+  for (int j = 0; j < s; ++j) {
+    if (rho * fabs(z[j]) <= tol) {
+      dval[nd] = d[j]; dcol[nd] = col[j]; ++nd;
+      continue;
+    }
+    if (pj < 0) { pj = j; continue; }
+    double S = z[pj], Cc = z[j];
+    double tau = hypot(Cc, S);
+    double t = d[j] - d[pj];
+    double c = Cc / tau, sn = -S / tau;
+    if (fabs(t * c * sn) <= tol) {
+      z[j] = tau;
+      z[pj] = 0.0;
+      rcol[2 * nrot] = col[pj];
+      rcol[2 * nrot + 1] = col[j];
+      rcs[2 * nrot] = c;
+      rcs[2 * nrot + 1] = sn;
+      ++nrot;
+      double tt = d[pj] * c * c + d[j] * sn * sn;
+      d[j] = d[pj] * sn * sn + d[j] * c * c;
+      d[pj] = tt;
+      dval[nd] = d[pj]; dcol[nd] = col[pj]; ++nd;
+      pj = j;
+    } else {
+      dl[k] = d[pj]; w[k] = z[pj]; gcol[k] = col[pj]; ++k;
+      pj = j;
+    }
+  }
+  if (pj >= 0) { dl[k] = d[pj]; w[k] = z[pj]; gcol[k] = col[pj]; ++k; }
+  *nd_out = nd;
+  *nrot_out = nrot;
+  return k;
+}

@@ -1,0 +1,1005 @@
+// Symmetric eigendecomposition with truncation, FP64 throughout: the
+// `eigen(Hermitian)` step of factorize (src/tensor/culinearalgebra.jl:74-108:
+// syevd, eigenvalues reversed to descending, truncate!, columns reversed and
+// sliced) used by replacebond! on the density matrix rho = phi phi^T.
+//
+// 1. Householder tridiagonalisation, blocked as in LAPACK dsytrd/dlatrd
+//    (panels of NB columns): per column a fused single-workgroup kernel
+//    (column update from the panel's V/W, reflector generation, finish of the
+//    previous w) and a symmetric matrix-vector product over the trailing
+//    block (one wavefront per column, coalesced, memory bound); per panel
+//    the rank-2*NB trailing update A -= V W^T + W V^T on FP64 MFMA.
+// 2. Divide and conquer on the tridiagonal (Cuppen tearing at every split,
+//    bottom-up merges of blocks 1,2,4,...): per merge a sort + deflation
+//    workgroup, rotation, secular-root, Gu-Eisenstat and eigenvector kernels,
+//    and Q_new = Q_old[:, nondeflated] V on MFMA (gathered columns through
+//    a device offset table) for large merges.
+// 3. Eigenvalues sorted descending, truncate! decided on the device, and only
+//    the k kept eigenvectors back-transformed by the compact-WY block
+//    reflectors (Schreiber & Van Loan 1989) on MFMA.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "dc_core.h"
+#include "gett.h"
+#include "itg_internal.h"
+#include "truncate_core.h"
+
+int itg_truncate_device_async(itg_ctx* ctx, const double* P, int64_t n, int64_t maxdim,
+                              int64_t mindim, double cutoff, int abs_cut, int rel_cut,
+                              int semantics, double* out3);
+
+namespace {
+
+constexpr int NB = 32;          // tridiagonalisation panel width
+constexpr int SMALL_MERGE = 256;
+
+__device__ double block_sum(double v, double* sh) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+  int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) sh[w] = v;
+  __syncthreads();
+  double r = 0.0;
+  int nw = (blockDim.x + 63) >> 6;
+  for (int i = 0; i < nw; ++i) r += sh[i];
+  return r;
+}
+
+// ------------------------------------------------------- tridiagonalisation
+__global__ __launch_bounds__(1024) void trd_step_kernel(double* __restrict__ A, int n, int p,
+                                                        int ifin, int icol,
+                                                        double* __restrict__ Vp,
+                                                        double* __restrict__ Wp,
+                                                        const double* __restrict__ y,
+                                                        double* d, double* e, double* tau) {
+  __shared__ double red[16];
+  __shared__ double xs[2 * NB];
+  __shared__ double sc[4];
+  const int t = threadIdx.x, nt = blockDim.x;
+  const size_t ld = (size_t)n;
+  // finish w of panel column ifin: w = tau (y - W x1 - V x2), w -= tau/2 (w.v) v
+  if (ifin >= 0) {
+    int jp = p + ifin;
+    double* w = Wp + (size_t)ifin * ld;
+    double tj = (jp < n - 1) ? tau[jp] : 0.0;
+    if (tj != 0.0) {
+      const double* v = Vp + (size_t)ifin * ld;
+      int lane = t & 63, wv = t >> 6, nw = nt >> 6;
+      for (int q = wv; q < 2 * ifin; q += nw) {
+        const double* col = (q < ifin) ? Vp + (size_t)q * ld : Wp + (size_t)(q - ifin) * ld;
+        double s = 0.0;
+        for (int r = jp + 1 + lane; r < n; r += 64) s += col[r] * v[r];
+        for (int o = 32; o > 0; o >>= 1) s += __shfl_down(s, o, 64);
+        if (lane == 0) xs[q] = s;
+      }
+      __syncthreads();
+      double part = 0.0;
+      for (int r = t; r < n; r += nt) {
+        double val = 0.0;
+        if (r > jp) {
+          double acc = y[r];
+          for (int q = 0; q < ifin; ++q)
+            acc -= Vp[r + (size_t)q * ld] * xs[ifin + q] + Wp[r + (size_t)q * ld] * xs[q];
+          val = tj * acc;
+          part += val * v[r];
+        }
+        w[r] = val;
+      }
+      double wdv = block_sum(part, red);
+      double a2 = -0.5 * tj * wdv;
+      for (int r = jp + 1 + t; r < n; r += nt) w[r] += a2 * v[r];
+    } else {
+      for (int r = t; r < n; r += nt) w[r] = 0.0;
+    }
+    __syncthreads();
+  }
+  // update column j = p + icol with the panel so far, then its reflector
+  if (icol >= 0) {
+    int j = p + icol;
+    double* a = A + (size_t)j * ld;
+    for (int q = t; q < icol; q += nt) {
+      xs[q] = Vp[j + (size_t)q * ld];
+      xs[NB + q] = Wp[j + (size_t)q * ld];
+    }
+    __syncthreads();
+    for (int r = j + t; r < n; r += nt) {
+      double acc = a[r];
+      for (int q = 0; q < icol; ++q)
+        acc -= Vp[r + (size_t)q * ld] * xs[NB + q] + Wp[r + (size_t)q * ld] * xs[q];
+      a[r] = acc;
+    }
+    __syncthreads();
+    if (t == 0) d[j] = a[j];
+    double* vcol = Vp + (size_t)icol * ld;
+    if (j < n - 1) {
+      double part = 0.0;
+      for (int r = j + 2 + t; r < n; r += nt) part += a[r] * a[r];
+      double xn2 = block_sum(part, red);
+      if (t == 0) {
+        double alpha = a[j + 1];
+        double beta = alpha, tv = 0.0, scale = 1.0;
+        if (xn2 != 0.0) {
+          beta = -copysign(sqrt(alpha * alpha + xn2), alpha);
+          tv = (beta - alpha) / beta;
+          scale = 1.0 / (alpha - beta);
+        }
+        sc[0] = scale;
+        e[j] = beta;
+        tau[j] = tv;
+      }
+      __syncthreads();
+      double scale = sc[0];
+      for (int r = t; r < n; r += nt) {
+        double v;
+        if (r <= j) v = 0.0;
+        else if (r == j + 1) v = 1.0;
+        else { v = a[r] * scale; a[r] = v; }
+        vcol[r] = v;
+      }
+      __syncthreads();
+      if (t == 0) a[j + 1] = 1.0;
+    } else {
+      if (t == 0) tau[j] = 0.0;
+      for (int r = t; r < n; r += nt) vcol[r] = 0.0;
+    }
+  }
+}
+
+// y[c] = sum_{r > j} A[r, c] v[r] for c > j (A symmetric in full storage)
+__global__ __launch_bounds__(256) void trd_symv_kernel(const double* __restrict__ A, int n, int j,
+                                                       const double* __restrict__ v,
+                                                       double* __restrict__ y) {
+  int lane = threadIdx.x & 63;
+  int c = j + 1 + blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (c >= n) return;
+  const double* col = A + (size_t)c * n;
+  double s0 = 0.0, s1 = 0.0;
+  int r = j + 1 + lane;
+  for (; r + 64 < n; r += 128) {
+    s0 += col[r] * v[r];
+    s1 += col[r + 64] * v[r + 64];
+  }
+  if (r < n) s0 += col[r] * v[r];
+  double s = s0 + s1;
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_down(s, o, 64);
+  if (lane == 0) y[c] = s;
+}
+
+// --------------------------------------------------------- back-transform
+// reflector j has v(j+off) = 1 and v(r) = A[r, j] below it (off = 1 for the
+// tridiagonal reduction, 0 for QR)
+__global__ void bt_build_v_kernel(const double* __restrict__ A, int n, int p, int nbp, int off,
+                                  double* __restrict__ Vb) {
+  int64_t tot = (int64_t)n * nbp;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < tot;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    int r = (int)(e % n), i = (int)(e / n);
+    int j = p + i;
+    double v = 0.0;
+    if (r == j + off) v = 1.0;
+    else if (r > j + off) v = A[r + (size_t)j * n];
+    Vb[e] = v;
+  }
+}
+
+// compact-WY triangular factor, forward columnwise (dlarft 'F','C')
+__global__ void bt_larft_kernel(int nbp, const double* __restrict__ G, const double* __restrict__ tau,
+                                double* __restrict__ T) {
+  __shared__ double Ts[NB][NB + 1];
+  __shared__ double tmp[NB];
+  int l = threadIdx.x;
+  for (int i = 0; i < NB; ++i) if (l < NB) Ts[l][i] = 0.0;
+  __syncthreads();
+  for (int i = 0; i < nbp; ++i) {
+    if (l < i) tmp[l] = -tau[i] * G[l + i * nbp];
+    __syncthreads();
+    if (l < i) {
+      double s = 0.0;
+      for (int m = l; m < i; ++m) s += Ts[l][m] * tmp[m];
+      Ts[l][i] = s;
+    }
+    if (l == i) Ts[i][i] = tau[i];
+    __syncthreads();
+  }
+  if (l < nbp)
+    for (int i = 0; i < nbp; ++i) T[l + i * nbp] = Ts[l][i];
+}
+
+// --------------------------------------------------------- divide & conquer
+__global__ void dc_init_kernel(int n, const double* d, const double* e, double* D, double* Q) {
+  int64_t tot = (int64_t)n * n;
+  for (int64_t x = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; x < tot;
+       x += (int64_t)gridDim.x * blockDim.x) {
+    int r = (int)(x % n), c = (int)(x / n);
+    Q[x] = (r == c) ? 1.0 : 0.0;
+    if (r == 0) {
+      double v = d[c];
+      if (c > 0) v -= fabs(e[c - 1]);
+      if (c < n - 1) v -= fabs(e[c]);
+      D[c] = v;
+    }
+  }
+}
+
+__device__ void bitonic_sort(double* key, int* val, int np2) {
+  for (int k = 2; k <= np2; k <<= 1) {
+    for (int jj = k >> 1; jj > 0; jj >>= 1) {
+      for (int i = threadIdx.x; i < np2; i += blockDim.x) {
+        int ixj = i ^ jj;
+        if (ixj > i) {
+          bool up = ((i & k) == 0);
+          double a = key[i], b = key[ixj];
+          if ((a > b) == up) {
+            key[i] = b;
+            key[ixj] = a;
+            int tv = val[i];
+            val[i] = val[ixj];
+            val[ixj] = tv;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+}
+
+struct DcBufs {
+  double *ds, *zs, *dl, *wv, *dval, *rcs, *tau, *zh, *mrho;
+  int *cs, *gcol, *dcol, *rcol, *org, *meta;
+};
+
+__global__ __launch_bounds__(1024) void dc_prepare_kernel(int n, int s_lvl, int half,
+                                                          const double* __restrict__ D,
+                                                          const double* __restrict__ Q,
+                                                          const double* __restrict__ e, DcBufs b) {
+  extern __shared__ double lds_d[];
+  int m = blockIdx.x;
+  int r0 = m * s_lvl;
+  int s = min(s_lvl, n - r0);
+  if (s <= half) return;
+  int s1 = half;
+  int np2 = 1;
+  while (np2 < s) np2 <<= 1;
+  double* key = lds_d;
+  int* val = (int*)(key + np2);
+  for (int i = threadIdx.x; i < np2; i += blockDim.x) {
+    key[i] = i < s ? D[r0 + i] : INFINITY;
+    val[i] = r0 + (i < s ? i : 0);
+  }
+  __syncthreads();
+  bitonic_sort(key, val, np2);
+  double beta = e[r0 + s1 - 1];
+  double sg = beta >= 0.0 ? 1.0 : -1.0;
+  for (int i = threadIdx.x; i < s; i += blockDim.x) {
+    int c = val[i];
+    double z = (c < r0 + s1) ? Q[(size_t)c * n + (r0 + s1 - 1)] : sg * Q[(size_t)c * n + (r0 + s1)];
+    b.zs[r0 + i] = z * 0.7071067811865476;
+    b.ds[r0 + i] = key[i];
+    b.cs[r0 + i] = c;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double rho = 2.0 * fabs(beta);
+    int nd = 0, nrot = 0;
+    int k = dc_deflate(s, b.ds + r0, b.zs + r0, b.cs + r0, rho, b.dl + r0, b.wv + r0,
+                       b.gcol + r0, b.dval + r0, b.dcol + r0, &nd, b.rcol + 2 * r0,
+                       b.rcs + 2 * r0, &nrot);
+    double w2 = 0.0;
+    for (int j = 0; j < k; ++j) w2 += b.wv[r0 + j] * b.wv[r0 + j];
+    b.meta[3 * m] = k;
+    b.meta[3 * m + 1] = nd;
+    b.meta[3 * m + 2] = nrot;
+    b.mrho[2 * m] = rho;
+    b.mrho[2 * m + 1] = w2;
+  }
+}
+
+__device__ __forceinline__ bool merge_range(int n, int s_lvl, int half, int m, int& r0, int& s) {
+  r0 = m * s_lvl;
+  s = min(s_lvl, n - r0);
+  return s > half;
+}
+
+__global__ void dc_rotate_kernel(int n, int s_lvl, int half, double* __restrict__ Q, DcBufs b) {
+  int m = blockIdx.x, r0, s;
+  if (!merge_range(n, s_lvl, half, m, r0, s)) return;
+  int nrot = b.meta[3 * m + 2];
+  int r = r0 + blockIdx.y * blockDim.x + threadIdx.x;
+  if (r >= r0 + s) return;
+  for (int t = 0; t < nrot; ++t) {
+    int pc = b.rcol[2 * r0 + 2 * t], qc = b.rcol[2 * r0 + 2 * t + 1];
+    double c = b.rcs[2 * r0 + 2 * t], sn = b.rcs[2 * r0 + 2 * t + 1];
+    double x = Q[(size_t)pc * n + r], yv = Q[(size_t)qc * n + r];
+    Q[(size_t)pc * n + r] = c * x + sn * yv;
+    Q[(size_t)qc * n + r] = c * yv - sn * x;
+  }
+}
+
+__global__ void dc_secular_kernel(int n, int s_lvl, int half, DcBufs b) {
+  int m = blockIdx.x, r0, s;
+  if (!merge_range(n, s_lvl, half, m, r0, s)) return;
+  int k = b.meta[3 * m];
+  int i = blockIdx.y * blockDim.x + threadIdx.x;
+  if (i >= k) return;
+  int o;
+  double tv;
+  dc_secular_root(i, k, b.dl + r0, b.wv + r0, b.mrho[2 * m], b.mrho[2 * m + 1], &o, &tv);
+  b.org[r0 + i] = o;
+  b.tau[r0 + i] = tv;
+}
+
+__global__ void dc_zhat_kernel(int n, int s_lvl, int half, DcBufs b) {
+  int m = blockIdx.x, r0, s;
+  if (!merge_range(n, s_lvl, half, m, r0, s)) return;
+  int k = b.meta[3 * m];
+  int i = blockIdx.y * blockDim.x + threadIdx.x;
+  if (i >= k) return;
+  b.zh[r0 + i] = dc_zhat(i, k, b.dl + r0, b.org + r0, b.tau + r0, b.wv + r0);
+}
+
+// column j of the rank-one eigenvector matrix, normalised (one wave per column)
+__global__ __launch_bounds__(256) void dc_vec_kernel(int n, int s_lvl, int half, double* __restrict__ Vb,
+                                                     DcBufs b) {
+  int m = blockIdx.x, r0, s;
+  if (!merge_range(n, s_lvl, half, m, r0, s)) return;
+  int k = b.meta[3 * m];
+  int lane = threadIdx.x & 63;
+  int j = blockIdx.y * 4 + (threadIdx.x >> 6);
+  if (j >= k) return;
+  const double* dl = b.dl + r0;
+  double lamo = dl[b.org[r0 + j]];
+  double tj = b.tau[r0 + j];
+  double s2 = 0.0;
+  for (int i = lane; i < k; i += 64) {
+    double v = b.zh[r0 + i] / ((dl[i] - lamo) - tj);
+    s2 += v * v;
+  }
+  for (int o = 32; o > 0; o >>= 1) s2 += __shfl_xor(s2, o, 64);
+  double inv = 1.0 / sqrt(s2);
+  double* Vm = Vb + (size_t)r0 * s_lvl;
+  for (int i = lane; i < k; i += 64)
+    Vm[i + (size_t)j * k] = b.zh[r0 + i] / ((dl[i] - lamo) - tj) * inv;
+}
+
+// new block columns: nondeflated (gemm part, small merges only), deflated
+// copies, and eigenvalues
+__global__ void dc_apply_kernel(int n, int s_lvl, int half, const double* __restrict__ Q,
+                                double* __restrict__ Qn, const double* __restrict__ Vb,
+                                double* __restrict__ Dn, DcBufs b, int do_gemm) {
+  int m = blockIdx.x, r0, s;
+  if (!merge_range(n, s_lvl, half, m, r0, s)) return;
+  int k = b.meta[3 * m];
+  int64_t tot = (int64_t)s * s;
+  for (int64_t e = (int64_t)blockIdx.y * blockDim.x + threadIdx.x; e < tot;
+       e += (int64_t)gridDim.y * blockDim.x) {
+    int r = (int)(e % s), jc = (int)(e / s);
+    if (jc < k) {
+      if (r == 0) Dn[r0 + jc] = b.dl[r0 + b.org[r0 + jc]] + b.tau[r0 + jc];
+      if (!do_gemm) continue;
+      const double* Vm = Vb + (size_t)r0 * s_lvl + (size_t)jc * k;
+      double acc = 0.0;
+      for (int t = 0; t < k; ++t) acc += Q[(size_t)b.gcol[r0 + t] * n + r0 + r] * Vm[t];
+      Qn[(size_t)(r0 + jc) * n + r0 + r] = acc;
+    } else {
+      int t2 = jc - k;
+      if (r == 0) Dn[r0 + jc] = b.dval[r0 + t2];
+      Qn[(size_t)(r0 + jc) * n + r0 + r] = Q[(size_t)b.dcol[r0 + t2] * n + r0 + r];
+    }
+  }
+}
+
+// a trailing block with no sibling at this level is carried over unchanged
+__global__ void dc_carry_kernel(int n, int r0, int s, const double* __restrict__ Q,
+                                double* __restrict__ Qn, const double* __restrict__ D,
+                                double* __restrict__ Dn) {
+  int64_t tot = (int64_t)s * s;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < tot;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    int r = (int)(e % s), c = (int)(e / s);
+    Qn[(size_t)(r0 + c) * n + r0 + r] = Q[(size_t)(r0 + c) * n + r0 + r];
+    if (r == 0) Dn[r0 + c] = D[r0 + c];
+  }
+}
+
+__global__ void dc_table_kernel(int k, const int* __restrict__ gcol, int n, int64_t* __restrict__ tab) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < k) tab[t] = (int64_t)gcol[t] * n;
+}
+
+__global__ __launch_bounds__(1024) void dc_final_sort_kernel(int n, const double* __restrict__ D,
+                                                             int* __restrict__ perm,
+                                                             double* __restrict__ wout) {
+  extern __shared__ double lds_d[];
+  int np2 = 1;
+  while (np2 < n) np2 <<= 1;
+  double* key = lds_d;
+  int* val = (int*)(key + np2);
+  for (int i = threadIdx.x; i < np2; i += blockDim.x) {
+    key[i] = i < n ? -D[i] : INFINITY;
+    val[i] = i < n ? i : 0;
+  }
+  __syncthreads();
+  bitonic_sort(key, val, np2);
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    perm[i] = val[i];
+    wout[i] = -key[i];
+  }
+}
+
+__global__ void gather_cols_kernel(int n, int k, const double* __restrict__ Q,
+                                   const int* __restrict__ perm, double* __restrict__ U) {
+  int64_t tot = (int64_t)n * k;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < tot;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    int r = (int)(e % n), c = (int)(e / n);
+    U[e] = Q[(size_t)perm[c] * n + r];
+  }
+}
+
+struct Carver {
+  char* base;
+  size_t off = 0;
+  template <typename T>
+  T* take(size_t count) {
+    off = (off + 255) & ~(size_t)255;
+    T* p = (T*)(base + off);
+    off += count * sizeof(T);
+    return p;
+  }
+};
+
+int set_lds_limit(itg_ctx* ctx) {
+  static bool done = false;
+  if (done) return ITG_OK;
+  ITG_HIP(ctx, hipFuncSetAttribute((const void*)dc_prepare_kernel,
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, 8192 * 12));
+  ITG_HIP(ctx, hipFuncSetAttribute((const void*)dc_final_sort_kernel,
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, 8192 * 12));
+  done = true;
+  return ITG_OK;
+}
+
+}  // namespace
+
+static int eig_workspace(itg_ctx* ctx, size_t bytes, void** out) {
+  if (bytes > ctx->eig_ws_bytes) {
+    if (ctx->eig_ws) {
+      ITG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+      ITG_HIP(ctx, hipFree(ctx->eig_ws));
+      ctx->eig_ws = nullptr;
+      ctx->eig_ws_bytes = 0;
+    }
+    if (hipMalloc(&ctx->eig_ws, bytes) != hipSuccess) {
+      hipGetLastError();
+      return itg_fail(ctx, ITG_EOOM, "eigensolver workspace of %zu bytes failed", bytes);
+    }
+    ctx->eig_ws_bytes = bytes;
+  }
+  *out = ctx->eig_ws;
+  return ITG_OK;
+}
+
+// Tridiagonalise A (n x n FP64, full symmetric storage) in place: d, e out;
+// reflectors left in A's strictly lower part (v_j(j+1) = 1 explicit), tau out.
+static int tridiagonalize(itg_ctx* ctx, int n, double* A, double* d, double* e, double* tau,
+                          double* Vp, double* Wp, double* y) {
+  for (int p = 0; p < n; p += NB) {
+    int nbp = std::min(NB, n - p);
+    for (int i = 0; i < nbp; ++i) {
+      int j = p + i;
+      trd_step_kernel<<<1, 1024, 0, ctx->stream>>>(A, n, p, i > 0 ? i - 1 : -1, i, Vp, Wp, y, d,
+                                                   e, tau);
+      ITG_LAUNCH_CHECK(ctx);
+      if (j < n - 1) {
+        int m = n - j - 1;
+        trd_symv_kernel<<<(m + 3) / 4, 256, 0, ctx->stream>>>(A, n, j, Vp + (size_t)i * n, y);
+        ITG_LAUNCH_CHECK(ctx);
+      }
+    }
+    trd_step_kernel<<<1, 1024, 0, ctx->stream>>>(A, n, p, nbp - 1, -1, Vp, Wp, y, d, e, tau);
+    ITG_LAUNCH_CHECK(ctx);
+    int q = p + nbp;
+    if (q < n) {
+      int m2 = n - q;
+      double* A22 = A + (size_t)q * n + q;
+      int rc = itg_gemm_internal(ctx, ITG_F64, ITG_MODE_NATIVE, 'N', 'T', m2, m2, nbp, -1.0,
+                                 Vp + q, n, Wp + q, n, 1.0, A22, n);
+      if (rc) return rc;
+      rc = itg_gemm_internal(ctx, ITG_F64, ITG_MODE_NATIVE, 'N', 'T', m2, m2, nbp, -1.0, Wp + q, n,
+                             Vp + q, n, 1.0, A22, n);
+      if (rc) return rc;
+    }
+  }
+  return ITG_OK;
+}
+
+// Eigen-decomposition of the tridiagonal (d, e): eigenvalues D (unsorted)
+// and eigenvectors as the columns of the returned buffer (Qa or Qb).
+static int tridiag_dc(itg_ctx* ctx, int n, const double* d, const double* e, double* Qa,
+                      double* Qb, double* Da, double* Db, double* Vb, DcBufs bufs, int64_t* tab,
+                      double** Qout, double** Dout) {
+  dc_init_kernel<<<itg_blocks((int64_t)n * n, 256, 16384), 256, 0, ctx->stream>>>(n, d, e, Da, Qa);
+  ITG_LAUNCH_CHECK(ctx);
+  double *Q = Qa, *Qn = Qb, *D = Da, *Dn = Db;
+  std::vector<int> meta;
+  for (int half = 1; half < n; half *= 2) {
+    int s_lvl = 2 * half;
+    int nm = (n + s_lvl - 1) / s_lvl;
+    size_t lds = (size_t)s_lvl * (sizeof(double) + sizeof(int));
+    dc_prepare_kernel<<<nm, 1024, lds, ctx->stream>>>(n, s_lvl, half, D, Q, e, bufs);
+    ITG_LAUNCH_CHECK(ctx);
+    meta.assign(3 * nm, 0);
+    ITG_HIP(ctx, hipMemcpyAsync(meta.data(), bufs.meta, 3 * nm * sizeof(int), hipMemcpyDeviceToHost,
+                                ctx->stream));
+    ITG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    dim3 g256(nm, (s_lvl + 255) / 256);
+    dc_rotate_kernel<<<g256, 256, 0, ctx->stream>>>(n, s_lvl, half, Q, bufs);
+    dc_secular_kernel<<<g256, 256, 0, ctx->stream>>>(n, s_lvl, half, bufs);
+    dc_zhat_kernel<<<g256, 256, 0, ctx->stream>>>(n, s_lvl, half, bufs);
+    dc_vec_kernel<<<dim3(nm, (s_lvl + 3) / 4), 256, 0, ctx->stream>>>(n, s_lvl, half, Vb, bufs);
+    ITG_LAUNCH_CHECK(ctx);
+    bool small = s_lvl <= SMALL_MERGE;
+    if (!small) {
+      for (int m = 0; m < nm; ++m) {
+        int r0 = m * s_lvl, s = std::min(s_lvl, n - r0), k = meta[3 * m];
+        if (s <= half || k == 0) continue;
+        dc_table_kernel<<<(k + 255) / 256, 256, 0, ctx->stream>>>(k, bufs.gcol + r0, n, tab);
+        // Qn[r0.., r0+j] (j < k) = sum_t Q[r0.., gcol[t]] V[t, j], computed as
+        // C^T(j, r) = V^T(j, t) A^T(t, r) so the stores run along rows
+        GettArgs g = {};
+        g.A = Vb + (size_t)r0 * s_lvl;
+        g.B = Q + r0;
+        g.C = Qn + (size_t)r0 * n + r0;
+        g.M = k; g.N = s; g.K = k;
+        g.sAm = k; g.sAk = 1;
+        g.sBk = -1; g.tBk = tab; g.sBn = 1;
+        g.sCm = n; g.sCn = 1;
+        g.alpha = 1.0; g.beta = 0.0;
+        int rc = itg_gett_launch(ctx, ITG_F64, ITG_MODE_NATIVE, &g, true, false);
+        if (rc) return rc;
+      }
+    }
+    dim3 gap(nm, (unsigned)std::min<int64_t>(((int64_t)s_lvl * s_lvl + 255) / 256, 8192));
+    dc_apply_kernel<<<gap, 256, 0, ctx->stream>>>(n, s_lvl, half, Q, Qn, Vb, Dn, bufs, small ? 1 : 0);
+    ITG_LAUNCH_CHECK(ctx);
+    int r0l = (nm - 1) * s_lvl, sl = n - r0l;
+    if (sl <= half) {
+      dc_carry_kernel<<<itg_blocks((int64_t)sl * sl, 256, 4096), 256, 0, ctx->stream>>>(n, r0l, sl, Q, Qn, D, Dn);
+      ITG_LAUNCH_CHECK(ctx);
+    }
+    std::swap(Q, Qn);
+    std::swap(D, Dn);
+  }
+  *Qout = Q;
+  *Dout = D;
+  return ITG_OK;
+}
+
+// U (n x k, FP64) <- Q U with Q = H_0 H_1 ... H_{nrefl-1}, reflectors stored
+// in A (ld n) as described at bt_build_v_kernel
+static int back_transform(itg_ctx* ctx, int n, int nrefl, int off, int k, const double* A,
+                          const double* tau, double* U, double* Vb, double* G, double* T,
+                          double* X, double* X2) {
+  if (nrefl < 1) return ITG_OK;
+  int last = ((nrefl - 1) / NB) * NB;
+  for (int p = last; p >= 0; p -= NB) {
+    int nbp = std::min(NB, nrefl - p);
+    int m2 = n - p - off;
+    if (nbp <= 0 || m2 <= 0) continue;
+    bt_build_v_kernel<<<itg_blocks((int64_t)n * nbp, 256, 4096), 256, 0, ctx->stream>>>(A, n, p, nbp, off, Vb);
+    ITG_LAUNCH_CHECK(ctx);
+    const double* V1 = Vb + (p + off);
+    int rc = itg_gemm_internal(ctx, ITG_F64, ITG_MODE_NATIVE, 'T', 'N', nbp, nbp, m2, 1.0, V1, n,
+                               V1, n, 0.0, G, nbp);
+    if (rc) return rc;
+    bt_larft_kernel<<<1, 64, 0, ctx->stream>>>(nbp, G, tau + p, T);
+    ITG_LAUNCH_CHECK(ctx);
+    rc = itg_gemm_internal(ctx, ITG_F64, ITG_MODE_NATIVE, 'T', 'N', nbp, k, m2, 1.0, V1, n,
+                           U + (p + off), n, 0.0, X, nbp);
+    if (rc) return rc;
+    rc = itg_gemm_internal(ctx, ITG_F64, ITG_MODE_NATIVE, 'N', 'N', nbp, k, nbp, 1.0, T, nbp, X,
+                           nbp, 0.0, X2, nbp);
+    if (rc) return rc;
+    rc = itg_gemm_internal(ctx, ITG_F64, ITG_MODE_NATIVE, 'N', 'N', m2, k, nbp, -1.0, V1, n, X2,
+                           nbp, 1.0, U + (p + off), n);
+    if (rc) return rc;
+  }
+  return ITG_OK;
+}
+
+extern "C" int itg_eigh_trunc(itg_ctx* ctx, int dtype, int64_t n64, void* Ain, double* w,
+                              void* Uout, int64_t maxdim, double cutoff, int abs_cut, int rel_cut,
+                              int64_t* kout, double* truncerr, double* docut) {
+  ITG_CHECK_CTX(ctx);
+  ITG_CHECK_DTYPE(ctx, dtype);
+  if (n64 < 0) return itg_fail(ctx, ITG_EDIM, "eigh: negative dimension");
+  if (n64 > 8192) return itg_fail(ctx, ITG_EUNSUPPORTED, "eigh: n = %lld > 8192", (long long)n64);
+  if (!kout) return itg_fail(ctx, ITG_EARG, "eigh: null k");
+  if (n64 == 0) {
+    *kout = 0;
+    if (truncerr) *truncerr = 0.0;
+    if (docut) *docut = 0.0;
+    return ITG_OK;
+  }
+  int rc = set_lds_limit(ctx);
+  if (rc) return rc;
+  int n = (int)n64;
+  size_t nn = (size_t)n * n;
+  bool f32 = dtype == ITG_F32;
+  size_t bytes = (3 * nn + (f32 ? 2 * nn : 0)) * sizeof(double) + 2 * (size_t)n * NB * 8 +
+                 (size_t)n * 8 * 24 + 4 * (size_t)n * NB * 8 + 4 * NB * NB * 8 + 64 * 1024 +
+                 64 * 256;
+  void* ws;
+  rc = eig_workspace(ctx, bytes, &ws);
+  if (rc) return rc;
+  Carver cv{(char*)ws};
+  double* Qa = cv.take<double>(nn);
+  double* Qb = cv.take<double>(nn);
+  double* Vbuf = cv.take<double>(nn);
+  double* A = f32 ? cv.take<double>(nn) : (double*)Ain;
+  double* U = f32 ? cv.take<double>(nn) : (double*)Uout;
+  double* Vp = cv.take<double>((size_t)n * NB);
+  double* Wp = cv.take<double>((size_t)n * NB);
+  double* y = cv.take<double>(n);
+  double* d = cv.take<double>(n);
+  double* e = cv.take<double>(n);
+  double* tau = cv.take<double>(n);
+  double* Da = cv.take<double>(n);
+  double* Db = cv.take<double>(n);
+  DcBufs b;
+  b.ds = cv.take<double>(n); b.zs = cv.take<double>(n); b.dl = cv.take<double>(n);
+  b.wv = cv.take<double>(n); b.dval = cv.take<double>(n); b.rcs = cv.take<double>(2 * (size_t)n);
+  b.tau = cv.take<double>(n); b.zh = cv.take<double>(n); b.mrho = cv.take<double>(2 * (size_t)n);
+  b.cs = cv.take<int>(n); b.gcol = cv.take<int>(n); b.dcol = cv.take<int>(n);
+  b.rcol = cv.take<int>(2 * (size_t)n); b.org = cv.take<int>(n); b.meta = cv.take<int>(3 * (size_t)n);
+  int* perm = cv.take<int>(n);
+  int64_t* tab = cv.take<int64_t>(n);
+  double* Gm = cv.take<double>(NB * NB);
+  double* Tm = cv.take<double>(NB * NB);
+  double* X = cv.take<double>((size_t)NB * n);
+  double* X2 = cv.take<double>((size_t)NB * n);
+  double* out3 = cv.take<double>(4);
+  if (cv.off > bytes) return itg_fail(ctx, ITG_EOOM, "eigh: workspace layout overflow");
+  if (f32) {
+    rc = itg_convert(ctx, (int64_t)nn, ITG_F32, Ain, ITG_F64, A);
+    if (rc) return rc;
+  }
+  // 1. tridiagonalise
+  rc = tridiagonalize(ctx, n, A, d, e, tau, Vp, Wp, y);
+  if (rc) return rc;
+  // 2. divide and conquer
+  double *Q, *D;
+  rc = tridiag_dc(ctx, n, d, e, Qa, Qb, Da, Db, Vbuf, b, tab, &Q, &D);
+  if (rc) return rc;
+  // 3. sort descending, truncate
+  {
+    int np2 = 1;
+    while (np2 < n) np2 <<= 1;
+    dc_final_sort_kernel<<<1, 1024, (size_t)np2 * 12, ctx->stream>>>(n, D, perm, w);
+    ITG_LAUNCH_CHECK(ctx);
+  }
+  int64_t k = n;
+  double te = 0.0, dc = 0.0;
+  if (cutoff >= 0.0 || (maxdim > 0 && maxdim < n)) {
+    rc = itg_truncate_device_async(ctx, w, n, maxdim, 1, cutoff < 0.0 ? 0.0 : cutoff, abs_cut,
+                                   rel_cut, ITG_TRUNC_NDTENSORS, out3);
+    if (rc) return rc;
+    double h[3];
+    ITG_HIP(ctx, hipMemcpyAsync(h, out3, 3 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    ITG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    k = (int64_t)h[0];
+    te = h[1];
+    dc = h[2];
+  }
+  // 4. kept eigenvectors: gather, back-transform
+  gather_cols_kernel<<<itg_blocks((int64_t)n * k, 256, 16384), 256, 0, ctx->stream>>>(n, (int)k, Q, perm, U);
+  ITG_LAUNCH_CHECK(ctx);
+  rc = back_transform(ctx, n, n - 1, 1, (int)k, A, tau, U, Vbuf, Gm, Tm, X, X2);
+  if (rc) return rc;
+  if (f32) {
+    rc = itg_convert(ctx, (int64_t)n * k, ITG_F64, U, ITG_F32, Uout);
+    if (rc) return rc;
+  }
+  *kout = k;
+  if (truncerr) *truncerr = te;
+  if (docut) *docut = dc;
+  return ITG_OK;
+}
+
+// ====================================================================== QR
+// Householder QR (culinearalgebra.jl:110-121): reflectors of W = A column by
+// column, then Q = H_0...H_{r-1} [I; 0] by the same compact-WY
+// back-transform, R = triu(W).
+namespace {
+
+__global__ __launch_bounds__(1024) void qr_reflect_kernel(double* __restrict__ W, int m, int j,
+                                                          double* __restrict__ tau) {
+  __shared__ double red[16];
+  __shared__ double sc;
+  double* a = W + (size_t)j * m;
+  double part = 0.0;
+  for (int r = j + 1 + threadIdx.x; r < m; r += blockDim.x) part += a[r] * a[r];
+  double xn2 = block_sum(part, red);
+  if (threadIdx.x == 0) {
+    double alpha = a[j], beta = alpha, tv = 0.0, scale = 1.0;
+    if (xn2 != 0.0) {
+      beta = -copysign(sqrt(alpha * alpha + xn2), alpha);
+      tv = (beta - alpha) / beta;
+      scale = 1.0 / (alpha - beta);
+    }
+    a[j] = beta;
+    tau[j] = tv;
+    sc = scale;
+  }
+  __syncthreads();
+  double scale = sc;
+  for (int r = j + 1 + threadIdx.x; r < m; r += blockDim.x) a[r] *= scale;
+}
+
+// W[j:m, c] -= tau_j v (v^T W[j:m, c]) for c > j, one wave per column
+__global__ __launch_bounds__(256) void qr_apply_kernel(double* __restrict__ W, int m, int n, int j,
+                                                       const double* __restrict__ tau) {
+  int lane = threadIdx.x & 63;
+  int c = j + 1 + blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (c >= n) return;
+  double tj = tau[j];
+  if (tj == 0.0) return;
+  const double* v = W + (size_t)j * m;
+  double* col = W + (size_t)c * m;
+  double s = 0.0;
+  for (int r = j + lane; r < m; r += 64) s += (r == j ? 1.0 : v[r]) * col[r];
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+  double f = tj * s;
+  for (int r = j + lane; r < m; r += 64) col[r] -= f * (r == j ? 1.0 : v[r]);
+}
+
+__global__ void qr_init_q_kernel(int m, int r, double* __restrict__ Q) {
+  int64_t tot = (int64_t)m * r;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < tot;
+       e += (int64_t)gridDim.x * blockDim.x)
+    Q[e] = ((e % m) == (e / m)) ? 1.0 : 0.0;
+}
+
+__global__ void qr_extract_r_kernel(int m, int n, int r, const double* __restrict__ W,
+                                    double* __restrict__ R) {
+  int64_t tot = (int64_t)r * n;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < tot;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    int i = (int)(e % r), c = (int)(e / r);
+    R[e] = i <= c ? W[i + (size_t)c * m] : 0.0;
+  }
+}
+
+// ------------------------------------------------------- one-sided Jacobi
+__global__ __launch_bounds__(256) void jac_round_kernel(double* __restrict__ W, int m,
+                                                        double* __restrict__ V, int nv, int ncols,
+                                                        int N, int rnd, double tol,
+                                                        double* __restrict__ offs) {
+  __shared__ double sh[3][4];
+  int i = blockIdx.x;
+  int p, q;
+  if (i == 0) { p = rnd; q = N - 1; }
+  else { p = (rnd + i) % (N - 1); q = (rnd - i + (N - 1)) % (N - 1); }
+  if (p >= ncols || q >= ncols) return;
+  double* a = W + (size_t)p * m;
+  double* b = W + (size_t)q * m;
+  double sa = 0.0, sb = 0.0, sg = 0.0;
+  for (int r = threadIdx.x; r < m; r += blockDim.x) {
+    double x = a[r], y = b[r];
+    sa += x * x; sb += y * y; sg += x * y;
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    sa += __shfl_down(sa, o, 64);
+    sb += __shfl_down(sb, o, 64);
+    sg += __shfl_down(sg, o, 64);
+  }
+  int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (lane == 0) { sh[0][w] = sa; sh[1][w] = sb; sh[2][w] = sg; }
+  __syncthreads();
+  sa = sh[0][0] + sh[0][1] + sh[0][2] + sh[0][3];
+  sb = sh[1][0] + sh[1][1] + sh[1][2] + sh[1][3];
+  sg = sh[2][0] + sh[2][1] + sh[2][2] + sh[2][3];
+  double rel = (sa > 0.0 && sb > 0.0) ? fabs(sg) / sqrt(sa * sb) : 0.0;
+  if (threadIdx.x == 0) offs[i] = fmax(offs[i], rel);
+  if (rel <= tol) return;
+  double zeta = (sb - sa) / (2.0 * sg);
+  double t = copysign(1.0, zeta) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
+  double c = 1.0 / sqrt(1.0 + t * t), s = c * t;
+  for (int r = threadIdx.x; r < m; r += blockDim.x) {
+    double x = a[r], y = b[r];
+    a[r] = c * x - s * y;
+    b[r] = s * x + c * y;
+  }
+  double* va = V + (size_t)p * nv;
+  double* vb = V + (size_t)q * nv;
+  for (int r = threadIdx.x; r < nv; r += blockDim.x) {
+    double x = va[r], y = vb[r];
+    va[r] = c * x - s * y;
+    vb[r] = s * x + c * y;
+  }
+}
+
+__global__ __launch_bounds__(256) void col_norm_kernel(const double* __restrict__ W, int m, int ncols,
+                                                       double* __restrict__ out) {
+  int lane = threadIdx.x & 63;
+  int c = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (c >= ncols) return;
+  const double* col = W + (size_t)c * m;
+  double s = 0.0;
+  for (int r = lane; r < m; r += 64) s += col[r] * col[r];
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+  if (lane == 0) out[c] = sqrt(s);
+}
+
+__global__ void square_kernel(int n, const double* __restrict__ x, double* __restrict__ y) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] = x[i] * x[i];
+}
+
+// Uo[:, i] = W[:, perm[i]] / s_i (zero column if s_i == 0), Vo[:, i] = V[:, perm[i]]
+__global__ void svd_gather_kernel(int m, int nv, int k, const double* __restrict__ W,
+                                  const double* __restrict__ V, const int* __restrict__ perm,
+                                  const double* __restrict__ s, double* __restrict__ Uo,
+                                  double* __restrict__ Vo, int ldu, int ldv) {
+  int64_t tot = (int64_t)(m > nv ? m : nv) * k;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < tot;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    int64_t rows = m > nv ? m : nv;
+    int r = (int)(e % rows), i = (int)(e / rows);
+    int c = perm[i];
+    if (r < m) Uo[r + (size_t)i * ldu] = s[i] > 0.0 ? W[r + (size_t)c * m] / s[i] : 0.0;
+    if (r < nv) Vo[r + (size_t)i * ldv] = V[r + (size_t)c * nv];
+  }
+}
+
+__global__ void transpose_kernel(int m, int n, const double* __restrict__ A, double* __restrict__ B) {
+  int64_t tot = (int64_t)m * n;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < tot;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    int i = (int)(e % m), j = (int)(e / m);
+    B[j + (size_t)i * n] = A[e];
+  }
+}
+
+}  // namespace
+
+extern "C" int itg_qr(itg_ctx* ctx, int dtype, int64_t m64, int64_t n64, const void* Ain, void* Qo,
+                      void* Ro) {
+  ITG_CHECK_CTX(ctx);
+  ITG_CHECK_DTYPE(ctx, dtype);
+  if (m64 < 0 || n64 < 0) return itg_fail(ctx, ITG_EDIM, "qr: negative dimension");
+  if (m64 * n64 > ((int64_t)1 << 31)) return itg_fail(ctx, ITG_EUNSUPPORTED, "qr: matrix too large");
+  if (m64 == 0 || n64 == 0) return ITG_OK;
+  int m = (int)m64, n = (int)n64, r = std::min(m, n);
+  bool f32 = dtype == ITG_F32;
+  size_t bytes = ((size_t)m * n + (size_t)m * r + (size_t)r * n + (size_t)m * NB + 2 * (size_t)NB * r +
+                  2 * NB * NB + (size_t)r + 64) * sizeof(double) + 16 * 256;
+  void* ws;
+  int rc = eig_workspace(ctx, bytes, &ws);
+  if (rc) return rc;
+  Carver cv{(char*)ws};
+  double* W = cv.take<double>((size_t)m * n);
+  double* Q = cv.take<double>((size_t)m * r);
+  double* R = cv.take<double>((size_t)r * n);
+  double* Vb = cv.take<double>((size_t)m * NB);
+  double* X = cv.take<double>((size_t)NB * r);
+  double* X2 = cv.take<double>((size_t)NB * r);
+  double* Gm = cv.take<double>(NB * NB);
+  double* Tm = cv.take<double>(NB * NB);
+  double* tau = cv.take<double>(r);
+  rc = itg_convert(ctx, (int64_t)m * n, dtype, Ain, ITG_F64, W);
+  if (rc) return rc;
+  for (int j = 0; j < r; ++j) {
+    qr_reflect_kernel<<<1, 1024, 0, ctx->stream>>>(W, m, j, tau);
+    if (j < n - 1)
+      qr_apply_kernel<<<(n - j - 1 + 3) / 4, 256, 0, ctx->stream>>>(W, m, n, j, tau);
+    ITG_LAUNCH_CHECK(ctx);
+  }
+  qr_init_q_kernel<<<itg_blocks((int64_t)m * r, 256, 8192), 256, 0, ctx->stream>>>(m, r, Q);
+  ITG_LAUNCH_CHECK(ctx);
+  rc = back_transform(ctx, m, r, 0, r, W, tau, Q, Vb, Gm, Tm, X, X2);
+  if (rc) return rc;
+  qr_extract_r_kernel<<<itg_blocks((int64_t)r * n, 256, 8192), 256, 0, ctx->stream>>>(m, n, r, W, R);
+  ITG_LAUNCH_CHECK(ctx);
+  rc = itg_convert(ctx, (int64_t)m * r, ITG_F64, Q, dtype, Qo);
+  if (rc) return rc;
+  return itg_convert(ctx, (int64_t)r * n, ITG_F64, R, dtype, Ro);
+  (void)f32;
+}
+
+extern "C" int itg_svd_trunc(itg_ctx* ctx, int dtype, int64_t m64, int64_t n64, void* Ain, void* Uo,
+                             double* So, void* Vo, int64_t maxdim, int64_t mindim, double cutoff,
+                             int abs_cut, int rel_cut, int64_t* kout, double* truncerr,
+                             double* docut) {
+  ITG_CHECK_CTX(ctx);
+  ITG_CHECK_DTYPE(ctx, dtype);
+  if (m64 < 0 || n64 < 0) return itg_fail(ctx, ITG_EDIM, "svd: negative dimension");
+  if (!kout) return itg_fail(ctx, ITG_EARG, "svd: null k");
+  if (m64 == 0 || n64 == 0) {
+    *kout = 0;
+    if (truncerr) *truncerr = 0.0;
+    if (docut) *docut = 0.0;
+    return ITG_OK;
+  }
+  if (std::min(m64, n64) > 8192 || m64 * n64 > ((int64_t)1 << 31))
+    return itg_fail(ctx, ITG_EUNSUPPORTED, "svd: matrix too large");
+  int rc = set_lds_limit(ctx);
+  if (rc) return rc;
+  int m = (int)m64, n = (int)n64;
+  bool tr = m < n;                 // work on A^T so that rows >= cols
+  int rows = tr ? n : m, cols = tr ? m : n;
+  int N = cols + (cols & 1);
+  size_t bytes = ((size_t)rows * cols * 2 + (size_t)cols * cols + (size_t)rows * cols +
+                  (size_t)cols * cols + 8 * (size_t)N + 64) * sizeof(double) + 16 * 256;
+  void* ws;
+  rc = eig_workspace(ctx, bytes, &ws);
+  if (rc) return rc;
+  Carver cv{(char*)ws};
+  double* A64 = cv.take<double>((size_t)m * n);
+  double* W = cv.take<double>((size_t)rows * cols);
+  double* V = cv.take<double>((size_t)cols * cols);
+  double* Ub = cv.take<double>((size_t)rows * cols);
+  double* Vbo = cv.take<double>((size_t)cols * cols);
+  double* offs = cv.take<double>(N);
+  double* sig = cv.take<double>(N);
+  double* sig_sorted = cv.take<double>(N);
+  double* P = cv.take<double>(N);
+  int* perm = cv.take<int>(N);
+  double* out3 = cv.take<double>(4);
+  rc = itg_convert(ctx, (int64_t)m * n, dtype, Ain, ITG_F64, A64);
+  if (rc) return rc;
+  if (tr) transpose_kernel<<<itg_blocks((int64_t)m * n, 256, 8192), 256, 0, ctx->stream>>>(m, n, A64, W);
+  else ITG_HIP(ctx, hipMemcpyAsync(W, A64, (size_t)m * n * 8, hipMemcpyDeviceToDevice, ctx->stream));
+  qr_init_q_kernel<<<itg_blocks((int64_t)cols * cols, 256, 8192), 256, 0, ctx->stream>>>(cols, cols, V);
+  ITG_LAUNCH_CHECK(ctx);
+  double tol = 4.0 * ITG_DC_EPS * std::sqrt((double)rows);
+  std::vector<double> hoffs(N / 2);
+  bool conv = cols < 2;
+  for (int sweep = 0; sweep < 60 && !conv; ++sweep) {
+    ITG_HIP(ctx, hipMemsetAsync(offs, 0, (N / 2) * sizeof(double), ctx->stream));
+    for (int rnd = 0; rnd < N - 1; ++rnd) {
+      jac_round_kernel<<<N / 2, 256, 0, ctx->stream>>>(W, rows, V, cols, cols, N, rnd, tol, offs);
+    }
+    ITG_LAUNCH_CHECK(ctx);
+    ITG_HIP(ctx, hipMemcpyAsync(hoffs.data(), offs, (N / 2) * sizeof(double),
+                                hipMemcpyDeviceToHost, ctx->stream));
+    ITG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    double mx = 0.0;
+    for (double x : hoffs) mx = std::max(mx, x);
+    conv = mx <= tol;
+  }
+  if (!conv) return itg_fail(ctx, ITG_ECONV, "svd: Jacobi sweeps did not converge");
+  col_norm_kernel<<<(cols + 3) / 4, 256, 0, ctx->stream>>>(W, rows, cols, sig);
+  int np2 = 1;
+  while (np2 < cols) np2 <<= 1;
+  dc_final_sort_kernel<<<1, 1024, (size_t)np2 * 12, ctx->stream>>>(cols, sig, perm, sig_sorted);
+  square_kernel<<<(cols + 255) / 256, 256, 0, ctx->stream>>>(cols, sig_sorted, P);
+  ITG_LAUNCH_CHECK(ctx);
+  rc = itg_truncate_device_async(ctx, P, cols, maxdim, mindim, cutoff, abs_cut, rel_cut,
+                                 ITG_TRUNC_NDTENSORS, out3);
+  if (rc) return rc;
+  double h[3];
+  ITG_HIP(ctx, hipMemcpyAsync(h, out3, 3 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  ITG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  int k = (int)h[0];
+  svd_gather_kernel<<<itg_blocks((int64_t)std::max(rows, cols) * k, 256, 8192), 256, 0, ctx->stream>>>(
+      rows, cols, k, W, V, perm, sig_sorted, Ub, Vbo, rows, cols);
+  ITG_LAUNCH_CHECK(ctx);
+  // A = Ub S Vbo^T (or its transpose)
+  double* Ua = tr ? Vbo : Ub;   // m x k, ld m
+  double* Va = tr ? Ub : Vbo;   // n x k, ld n
+  rc = itg_convert(ctx, (int64_t)m * k, ITG_F64, Ua, dtype, Uo);
+  if (rc) return rc;
+  rc = itg_convert(ctx, (int64_t)n * k, ITG_F64, Va, dtype, Vo);
+  if (rc) return rc;
+  ITG_HIP(ctx, hipMemcpyAsync(So, sig_sorted, std::min(m, n) * sizeof(double), hipMemcpyDeviceToDevice,
+                              ctx->stream));
+  *kout = k;
+  if (truncerr) *truncerr = h[1];
+  if (docut) *docut = h[2];
+  return ITG_OK;
+}

@@ -33,24 +33,7 @@ typedef double d4_t __attribute__((ext_vector_type(4)));
 typedef float f4_t __attribute__((ext_vector_type(4)));
 typedef __bf16 bf8_t __attribute__((ext_vector_type(8)));
 
-struct GettArgs {
-  const void* A;
-  const void* B;
-  void* C;
-  int64_t M, N, K;
-  // affine strides (>= 0) or -1 to use the table
-  int64_t sAm, sAk, sBk, sBn, sCm, sCn;
-  const int64_t* tAm;
-  const int64_t* tAk;
-  const int64_t* tBk;
-  const int64_t* tBn;
-  const int64_t* tCm;
-  const int64_t* tCn;
-  double alpha, beta;
-  int64_t kchunk;   // K range per blockIdx.z
-  void* partial;    // split-K partials [z][n][m] in the compute type, or null
-  int64_t tilesM, tilesN;
-};
+#include "gett.h"
 
 __device__ __forceinline__ int64_t goff(const int64_t* t, int64_t s, int64_t i) {
   return s >= 0 ? i * s : t[i];
@@ -733,4 +716,9 @@ extern "C" int itg_gemm(itg_ctx* ctx, int dtype, int mode, char ta, char tb, int
   if (m < 0 || n < 0 || k < 0 || ldc < std::max<int64_t>(1, m))
     return itg_fail(ctx, ITG_EDIM, "bad GEMM dimensions");
   return itg_gemm_internal(ctx, dtype, mode, ta, tb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
+}
+
+int itg_gett_launch(itg_ctx* ctx, int dtype, int mode, GettArgs* g, bool ak, bool bk) {
+  if (g->M <= 0 || g->N <= 0) return ITG_OK;
+  return launch_gett(ctx, dtype, mode, *g, ak, bk);
 }

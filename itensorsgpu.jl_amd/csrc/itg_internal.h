@@ -24,6 +24,9 @@ struct itg_ctx {
   // grow-only device workspace for internal temporaries
   void* ws = nullptr;
   size_t ws_bytes = 0;
+  // grow-only device workspace of the factorisations (eigh/svd/qr)
+  void* eig_ws = nullptr;
+  size_t eig_ws_bytes = 0;
   // small device scratch for reductions + pinned host mirror
   double* red = nullptr;        // device, kRedSlots doubles
   double* red_host = nullptr;   // pinned host, kRedSlots doubles

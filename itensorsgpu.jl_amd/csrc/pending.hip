@@ -2,27 +2,6 @@
 // units; each returns ITG_EUNSUPPORTED until implemented.
 #include "itg_internal.h"
 
-extern "C" int itg_eigh_trunc(itg_ctx* ctx, int, int64_t, void*, double*, void*, int64_t, double,
-                              int, int, int64_t*, double*, double*) {
-  return itg_fail(ctx, ITG_EUNSUPPORTED, "itg_eigh_trunc not built");
-}
-extern "C" int itg_svd_trunc(itg_ctx* ctx, int, int64_t, int64_t, void*, void*, double*, void*,
-                             int64_t, int64_t, double, int, int, int64_t*, double*, double*) {
-  return itg_fail(ctx, ITG_EUNSUPPORTED, "itg_svd_trunc not built");
-}
-extern "C" int itg_qr(itg_ctx* ctx, int, int64_t, int64_t, const void*, void*, void*) {
-  return itg_fail(ctx, ITG_EUNSUPPORTED, "itg_qr not built");
-}
-extern "C" int64_t itg_heff2_work_elems(int64_t, int64_t, int64_t, int64_t) { return 0; }
-extern "C" int itg_heff2_apply(itg_ctx* ctx, int, int, int64_t, int64_t, int64_t, int64_t,
-                               const void*, const void*, const void*, const void*, const void*,
-                               void*, void*) {
-  return itg_fail(ctx, ITG_EUNSUPPORTED, "itg_heff2_apply not built");
-}
-extern "C" int itg_heff2_diag(itg_ctx* ctx, int, int64_t, int64_t, int64_t, int64_t, const void*,
-                              const void*, const void*, const void*, void*) {
-  return itg_fail(ctx, ITG_EUNSUPPORTED, "itg_heff2_diag not built");
-}
 extern "C" int64_t itg_mlp_param_count(int i, int h, int o) {
   return (int64_t)h * i + h + (int64_t)h * h + h + (int64_t)o * h + o;
 }

@@ -81,11 +81,9 @@ _SIGS = {
                                 C.c_int64, C.c_int64, C.c_double, C.c_int, C.c_int, _i64p, _dp,
                                 _dp]),
     "itg_qr": (C.c_int, [_ctxp, C.c_int, C.c_int64, C.c_int64, _vp, _vp, _vp]),
-    "itg_heff2_work_elems": (C.c_int64, [C.c_int64, C.c_int64, C.c_int64, C.c_int64]),
-    "itg_heff2_apply": (C.c_int, [_ctxp, C.c_int, C.c_int, C.c_int64, C.c_int64, C.c_int64,
-                                  C.c_int64, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
-    "itg_heff2_diag": (C.c_int, [_ctxp, C.c_int, C.c_int64, C.c_int64, C.c_int64, C.c_int64,
-                                 _vp, _vp, _vp, _vp, _vp]),
+    "itg_heff2_work_elems": (C.c_int64, [C.c_int64] * 6),
+    "itg_heff2_apply": (C.c_int, [_ctxp, C.c_int, C.c_int] + [C.c_int64] * 6 + [_vp] * 7),
+    "itg_heff2_diag": (C.c_int, [_ctxp, C.c_int] + [C.c_int64] * 6 + [_vp] * 5),
     "itg_mlp_param_count": (C.c_int64, [C.c_int, C.c_int, C.c_int]),
     "itg_mlp_forward": (C.c_int, [_ctxp, C.c_int, C.c_int, C.c_int, C.c_int, _vp, _vp, _vp,
                                   _vp]),

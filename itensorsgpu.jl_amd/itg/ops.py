@@ -228,27 +228,29 @@ def qr(A, m, n):
 
 
 # ------------------------------------------------------------------- Heff
-def heff2_work(chiL, w, d, chiR, dtype="float64"):
-    return empty(lib().itg_heff2_work_elems(chiL, w, d, chiR), dtype)
+def heff2_work(chiL, wl, wm, wr, d, chiR, dtype="float64"):
+    return empty(lib().itg_heff2_work_elems(chiL, wl, wm, wr, d, chiR), dtype)
 
 
-def heff2_apply(L, W1, W2, R, psi, chiL, w, d, chiR, mode="native", out=None, work=None):
+def heff2_apply(L, W1, W2, R, psi, chiL, wl, wm, wr, d, chiR, mode="native", out=None,
+                work=None):
+    """ProjMPO two-site product: out = Heff psi (psi, out: chiL x d x d x chiR)."""
     dt = str(psi.dtype).split(".")[-1]
     if out is None:
         out = empty(chiL * d * d * chiR, dt)
     if work is None:
-        work = heff2_work(chiL, w, d, chiR, dt)
+        work = heff2_work(chiL, wl, wm, wr, d, chiR, dt)
     c = ctx()
-    c.check(lib().itg_heff2_apply(c.ptr, dtype_code(psi), _lib.MODES[mode], chiL, w, d, chiR,
-                                  ptr(L), ptr(W1), ptr(W2), ptr(R), ptr(psi), ptr(out),
+    c.check(lib().itg_heff2_apply(c.ptr, dtype_code(psi), _lib.MODES[mode], chiL, wl, wm, wr, d,
+                                  chiR, ptr(L), ptr(W1), ptr(W2), ptr(R), ptr(psi), ptr(out),
                                   ptr(work)))
     return out
 
 
-def heff2_diag(L, W1, W2, R, chiL, w, d, chiR):
+def heff2_diag(L, W1, W2, R, chiL, wl, wm, wr, d, chiR):
     dt = str(L.dtype).split(".")[-1]
     out = empty(chiL * d * d * chiR, dt)
     c = ctx()
-    c.check(lib().itg_heff2_diag(c.ptr, dtype_code(L), chiL, w, d, chiR, ptr(L), ptr(W1),
-                                 ptr(W2), ptr(R), ptr(out)))
+    c.check(lib().itg_heff2_diag(c.ptr, dtype_code(L), chiL, wl, wm, wr, d, chiR, ptr(L),
+                                 ptr(W1), ptr(W2), ptr(R), ptr(out)))
     return out

@@ -1,0 +1,195 @@
+"""GPU parity: effective-Hamiltonian apply/diagonal, truncated eigen, SVD and
+QR through the C ABI against the CPU oracle (numpy LAPACK) on seeded inputs."""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import dmrg as odmrg
+from oracle import linalg as olin
+from oracle import tensor_ops as ops
+from oracle.truncate import ndtensors_truncate
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def dev():
+    import itg
+    from itg import ops as d
+    itg._lib.ctx()
+    return d
+
+
+def up(dev, a, dtype="float64"):
+    return dev.to_device(ops.to_flat(a), dtype)
+
+
+def rel(a, b):
+    return float(np.linalg.norm((a - b).ravel()) / max(np.linalg.norm(b.ravel()), 1e-300))
+
+
+def sym_env(rng, chi, w):
+    L = rng.standard_normal((chi, w, chi))
+    return L + np.transpose(L, (2, 1, 0))
+
+
+# --------------------------------------------------------------------- Heff
+@pytest.mark.parametrize("chiL,wl,wm,wr,d,chiR", [(16, 1, 5, 5, 2, 24), (40, 5, 5, 5, 2, 40),
+                                                  (33, 5, 5, 1, 2, 17), (64, 14, 14, 14, 2, 48),
+                                                  (20, 5, 5, 5, 3, 20), (257, 5, 5, 5, 2, 130)])
+def test_heff2_apply_matches_oracle(dev, chiL, wl, wm, wr, d, chiR):
+    rng = np.random.default_rng(chiL + 7 * wl)
+    L = sym_env(rng, chiL, wl)
+    R = sym_env(rng, chiR, wr)
+    W1 = rng.standard_normal((wl, wm, d, d))
+    W2 = rng.standard_normal((wm, wr, d, d))
+    psi = rng.standard_normal((chiL, d, d, chiR))
+    want = odmrg.heff_apply(L, W1, W2, R, psi)
+    for mode, tol in [("native", 1e-13), ("fp32", 2e-5), ("bf16x3", 5e-5)]:
+        y = dev.heff2_apply(up(dev, L), up(dev, W1), up(dev, W2), up(dev, R), up(dev, psi),
+                            chiL, wl, wm, wr, d, chiR, mode=mode)
+        err = rel(ops.from_flat(dev.to_host(y), want.shape), want)
+        print(mode, (chiL, wl, wm, wr, d, chiR), err)
+        assert err < tol
+    dg = dev.heff2_diag(up(dev, L), up(dev, W1), up(dev, W2), up(dev, R), chiL, wl, wm, wr, d, chiR)
+    np.testing.assert_allclose(ops.from_flat(dev.to_host(dg), want.shape),
+                               odmrg.heff_diag(L, W1, W2, R), rtol=1e-12, atol=1e-12)
+
+
+def test_heff2_config2_chi1024(dev):
+    """Effective-Hamiltonian microbench shapes: chi=1024, w=5, d=2."""
+    rng = np.random.default_rng(2)
+    chi, w, d = 1024, 5, 2
+    L = rng.standard_normal((chi, w, chi))
+    R = rng.standard_normal((chi, w, chi))
+    W1 = rng.standard_normal((w, w, d, d))
+    W2 = rng.standard_normal((w, w, d, d))
+    psi = rng.standard_normal((chi, d, d, chi))
+    want = odmrg.heff_apply(L, W1, W2, R, psi)
+    for mode, tol in [("native", 1e-13), ("fp32", 2e-5), ("bf16x3", 5e-5)]:
+        y = dev.heff2_apply(up(dev, L), up(dev, W1), up(dev, W2), up(dev, R), up(dev, psi),
+                            chi, w, w, w, d, chi, mode=mode)
+        err = rel(ops.from_flat(dev.to_host(y), want.shape), want)
+        print("chi=1024", mode, err)
+        assert err < tol
+
+
+# -------------------------------------------------------------------- eigen
+def density_matrix(rng, n, decades=14):
+    lam = 10.0 ** (-np.linspace(0, decades, n)) * rng.uniform(0.5, 1.0, n)
+    Q, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    return (Q * lam) @ Q.T
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 31, 32, 33, 100, 257, 600])
+def test_eigh_full_spectrum(dev, n):
+    rng = np.random.default_rng(n)
+    A = rng.standard_normal((n, n))
+    A = A + A.T
+    w, U, k, te, dc = dev.eigh_trunc(up(dev, A), n)
+    assert k == n
+    wh = dev.to_host(w)
+    Uh = ops.from_flat(dev.to_host(U), (n, n))
+    wr = np.linalg.eigvalsh(A)[::-1]
+    scale = np.abs(wr).max()
+    assert np.abs(wh - wr).max() <= 1e-13 * scale * max(1, n / 64)
+    assert np.abs(Uh.T @ Uh - np.eye(n)).max() < 1e-12
+    assert np.abs(A @ Uh - Uh * wh).max() < 1e-12 * scale * max(1, n / 64)
+
+
+@pytest.mark.parametrize("n,maxdim,cutoff", [(200, 100, 1e-10), (512, 512, 1e-12), (1024, 300, 1e-8)])
+def test_eigh_truncated_density_matrix(dev, n, maxdim, cutoff):
+    rng = np.random.default_rng(n + 1)
+    rho = density_matrix(rng, n)
+    w, U, k, te, dc = dev.eigh_trunc(up(dev, rho), n, maxdim=maxdim, cutoff=cutoff)
+    wr, Ur, ter, dcr = olin.eigh_trunc(rho, maxdim=maxdim, cutoff=cutoff)
+    assert k == Ur.shape[1]
+    assert math.isclose(te, ter, rel_tol=1e-8, abs_tol=1e-20)
+    wh = dev.to_host(w)[:k]
+    assert np.abs(wh - wr).max() <= 1e-14 * wr[0] * 10
+    Uh = ops.from_flat(dev.to_host(U)[: n * k], (n, k))
+    # eigenvectors up to sign, for well-separated eigenvalues
+    gaps = np.abs(np.diff(wr))
+    ok = np.ones(k, bool)
+    ok[:-1] &= gaps > 1e-6 * wr[0]
+    ok[1:] &= gaps > 1e-6 * wr[0]
+    dots = np.abs(np.sum(Uh * Ur, axis=0))
+    assert np.all(np.abs(dots[ok] - 1) < 1e-8)
+    assert np.abs(Uh.T @ Uh - np.eye(k)).max() < 1e-12
+
+
+def test_eigh_degenerate_multiplets(dev):
+    """Exactly degenerate eigenvalues (SU(2) multiplets in Heisenberg density
+    matrices): the kept invariant subspace must be exact."""
+    rng = np.random.default_rng(9)
+    lam = np.repeat(10.0 ** -np.arange(0, 12, 1.0), [1, 3, 5, 3, 1, 3, 5, 7, 3, 1, 3, 5])
+    n = lam.size
+    Q, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    rho = (Q * lam) @ Q.T
+    w, U, k, te, dc = dev.eigh_trunc(up(dev, rho), n, maxdim=n, cutoff=1e-12)
+    Uh = ops.from_flat(dev.to_host(U)[: n * k], (n, k))
+    P = Uh @ Uh.T
+    Pr = Q[:, :k] @ Q[:, :k].T
+    assert np.abs(P - Pr).max() < 1e-9
+    assert np.abs(Uh.T @ Uh - np.eye(k)).max() < 1e-12
+
+
+def test_eigh_fp32_storage(dev):
+    rng = np.random.default_rng(10)
+    n = 150
+    A = rng.standard_normal((n, n))
+    A = A + A.T
+    w, U, k, te, dc = dev.eigh_trunc(up(dev, A, "float32"), n)
+    wr = np.linalg.eigvalsh(A.astype(np.float32).astype(np.float64))[::-1]
+    assert np.abs(dev.to_host(w) - wr).max() < 1e-12 * np.abs(wr).max()
+
+
+# ---------------------------------------------------------------------- svd
+@pytest.mark.parametrize("m,n", [(15, 8), (8, 15), (40, 40), (100, 37), (1, 5), (5, 1)])
+def test_svd_invariants(dev, m, n):
+    """test/test_cuitensor.jl:105-112: A = U S V^T, U^T U = I, V^T V = I
+    (rtol 1e-14), singular values against LAPACK."""
+    rng = np.random.default_rng(m * 100 + n)
+    A = rng.standard_normal((m, n))
+    U, S, V, k, te, dc = dev.svd_trunc(up(dev, A), m, n)
+    r = min(m, n)
+    assert k == r
+    Uh = ops.from_flat(dev.to_host(U)[: m * k], (m, k))
+    Vh = ops.from_flat(dev.to_host(V)[: n * k], (n, k))
+    Sh = dev.to_host(S)[:k]
+    np.testing.assert_allclose(Uh @ np.diag(Sh) @ Vh.T, A, atol=1e-13 * np.abs(A).max() * 10)
+    assert np.abs(Uh.T @ Uh - np.eye(k)).max() < 1e-14 * 10
+    assert np.abs(Vh.T @ Vh - np.eye(k)).max() < 1e-14 * 10
+    np.testing.assert_allclose(Sh, np.linalg.svd(A, compute_uv=False), rtol=1e-13)
+
+
+def test_svd_truncation(dev):
+    rng = np.random.default_rng(3)
+    m, n = 60, 50
+    U0, _ = np.linalg.qr(rng.standard_normal((m, n)))
+    V0, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    s = 10.0 ** -np.linspace(0, 10, n)
+    A = (U0 * s) @ V0.T
+    U, S, V, k, te, dc = dev.svd_trunc(up(dev, A), m, n, maxdim=40, cutoff=1e-12)
+    kr, ter, dcr = ndtensors_truncate(s ** 2, maxdim=40, cutoff=1e-12)
+    assert k == kr
+    assert math.isclose(te, ter, rel_tol=1e-6, abs_tol=1e-24)
+    np.testing.assert_allclose(dev.to_host(S)[:k], s[:k], rtol=1e-10)
+
+
+# ----------------------------------------------------------------------- qr
+@pytest.mark.parametrize("m,n", [(20, 12), (12, 20), (64, 64), (300, 130), (1, 1)])
+def test_qr(dev, m, n):
+    """test/test_cuitensor.jl:125-130: A = QR, Q^T Q = I (atol 1e-14)."""
+    rng = np.random.default_rng(m + n)
+    A = rng.standard_normal((m, n))
+    Q, R = dev.qr(up(dev, A), m, n)
+    r = min(m, n)
+    Qh = ops.from_flat(dev.to_host(Q), (m, r))
+    Rh = ops.from_flat(dev.to_host(R), (r, n))
+    np.testing.assert_allclose(Qh @ Rh, A, atol=1e-13 * max(1, np.abs(A).max()))
+    assert np.abs(Qh.T @ Qh - np.eye(r)).max() < 1e-14 * 10
+    assert np.allclose(np.tril(Rh, -1), 0)
+    Qr, Rr = np.linalg.qr(A)
+    np.testing.assert_allclose(np.abs(np.diag(Rh)), np.abs(np.diag(Rr)), rtol=1e-12)
