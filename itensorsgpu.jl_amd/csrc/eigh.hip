@@ -210,12 +210,16 @@ __global__ void bt_larft_kernel(int nbp, const double* __restrict__ G, const dou
 }
 
 // --------------------------------------------------------- divide & conquer
-__global__ void dc_init_kernel(int n, const double* d, const double* e, double* D, double* Q) {
+// Q = I and Qn = 0: every merge rewrites whole block squares, and entries
+// outside the current squares of both ping-pong buffers must stay zero
+__global__ void dc_init_kernel(int n, const double* d, const double* e, double* D, double* Q,
+                               double* Qn) {
   int64_t tot = (int64_t)n * n;
   for (int64_t x = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; x < tot;
        x += (int64_t)gridDim.x * blockDim.x) {
     int r = (int)(x % n), c = (int)(x / n);
     Q[x] = (r == c) ? 1.0 : 0.0;
+    Qn[x] = 0.0;
     if (r == 0) {
       double v = d[c];
       if (c > 0) v -= fabs(e[c - 1]);
@@ -522,7 +526,7 @@ static int tridiagonalize(itg_ctx* ctx, int n, double* A, double* d, double* e, 
 static int tridiag_dc(itg_ctx* ctx, int n, const double* d, const double* e, double* Qa,
                       double* Qb, double* Da, double* Db, double* Vb, DcBufs bufs, int64_t* tab,
                       double** Qout, double** Dout) {
-  dc_init_kernel<<<itg_blocks((int64_t)n * n, 256, 16384), 256, 0, ctx->stream>>>(n, d, e, Da, Qa);
+  dc_init_kernel<<<itg_blocks((int64_t)n * n, 256, 16384), 256, 0, ctx->stream>>>(n, d, e, Da, Qa, Qb);
   ITG_LAUNCH_CHECK(ctx);
   double *Q = Qa, *Qn = Qb, *D = Da, *Dn = Db;
   std::vector<int> meta;
