@@ -110,6 +110,11 @@ int itg_scal(itg_ctx* ctx, int dtype, int64_t n, double alpha, void* x);
 /* y = alpha*x + beta*y. */
 int itg_axpby(itg_ctx* ctx, int dtype, int64_t n, double alpha, const void* x,
               double beta, void* y);
+/* Davidson diagonal preconditioner t = q ./ (diag - lambda), where
+ * |diag - lambda| below `floor` is replaced by floor (sign kept). No
+ * counterpart in the reference, whose davidson is unpreconditioned. */
+int itg_precond_diag(itg_ctx* ctx, int dtype, int64_t n, const void* q,
+                     const void* diag, double lambda, double floor, void* t);
 /* FP64-accumulated dot and 2-norm returned to the host (cudense.jl:25-27). */
 int itg_dot(itg_ctx* ctx, int dtype, int64_t n, const void* x, const void* y,
             double* out);

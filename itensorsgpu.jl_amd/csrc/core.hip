@@ -290,6 +290,33 @@ extern "C" int itg_axpby(itg_ctx* ctx, int dtype, int64_t n, double alpha, const
   return ITG_OK;
 }
 
+template <typename T>
+__global__ void precond_diag_kernel(int64_t n, const T* q, const T* dg, double lam, double fl,
+                                    T* t) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    double den = (double)dg[i] - lam;
+    if (fabs(den) < fl) den = den < 0.0 ? -fl : fl;
+    t[i] = (T)((double)q[i] / den);
+  }
+}
+
+extern "C" int itg_precond_diag(itg_ctx* ctx, int dtype, int64_t n, const void* q,
+                                const void* diag, double lambda, double floor, void* t) {
+  ITG_CHECK_CTX(ctx);
+  ITG_CHECK_DTYPE(ctx, dtype);
+  if (n <= 0) return ITG_OK;
+  if (!(floor > 0.0)) return itg_fail(ctx, ITG_EARG, "precond_diag: floor must be > 0");
+  if (dtype == ITG_F64)
+    precond_diag_kernel<double><<<itg_blocks(n, 256, 4096), 256, 0, ctx->stream>>>(
+        n, (const double*)q, (const double*)diag, lambda, floor, (double*)t);
+  else
+    precond_diag_kernel<float><<<itg_blocks(n, 256, 4096), 256, 0, ctx->stream>>>(
+        n, (const float*)q, (const float*)diag, lambda, floor, (float*)t);
+  ITG_LAUNCH_CHECK(ctx);
+  return ITG_OK;
+}
+
 // block-level FP64 sum of one value per thread (256 threads)
 __device__ __forceinline__ double block_sum_256(double v) {
   __shared__ double sh[4];

@@ -61,6 +61,8 @@ _SIGS = {
                                    C.c_double, _vp]),
     "itg_scal": (C.c_int, [_ctxp, C.c_int, C.c_int64, C.c_double, _vp]),
     "itg_axpby": (C.c_int, [_ctxp, C.c_int, C.c_int64, C.c_double, _vp, C.c_double, _vp]),
+    "itg_precond_diag": (C.c_int, [_ctxp, C.c_int, C.c_int64, _vp, _vp, C.c_double, C.c_double,
+                                   _vp]),
     "itg_dot": (C.c_int, [_ctxp, C.c_int, C.c_int64, _vp, _vp, _dp]),
     "itg_nrm2": (C.c_int, [_ctxp, C.c_int, C.c_int64, _vp, _dp]),
     "itg_contract": (C.c_int, [_ctxp, C.c_int, C.c_int,

@@ -1,0 +1,319 @@
+"""ITensor-level host mirror over libitg: Index, ITensor with `roc`/`cpu`
+storage conversion, contraction by index identity, permutation, addition,
+norm/scalar, and the order-2 factorisations used by ITensors (svd, eigen,
+qr, factorize).
+
+Mirrors the reference's ITensor-level GPU surface: `cuITensor`/`cu`/`cpu`
+(src/cuitensor.jl:1-33), `randomCuITensor` (src/cuitensor.jl:35-49),
+`permute`, `*`, `+`, `-` (src/tensor/cudense.jl:83-110,333-500) and
+`svd`/`eigen`/`qr` (src/tensor/culinearalgebra.jl:33-121), with `roc` in place
+of `cu`. Compute runs only on device storage: operations on host tensors
+raise (there is no CPU path here).
+"""
+import itertools
+
+import numpy as np
+
+from . import _lib
+from . import ops
+
+_ids = itertools.count(1)
+
+
+class Index:
+    """Index identity = (id, plev); tags are labels only."""
+
+    __slots__ = ("dim", "id", "tags", "plev")
+
+    def __init__(self, dim, tags="", plev=0, id=None):
+        self.dim = int(dim)
+        self.id = next(_ids) if id is None else id
+        self.tags = tags
+        self.plev = plev
+
+    def prime(self, n=1):
+        return Index(self.dim, self.tags, self.plev + n, self.id)
+
+    def noprime(self):
+        return Index(self.dim, self.tags, 0, self.id)
+
+    def sim(self):
+        return Index(self.dim, self.tags, self.plev)
+
+    def __eq__(self, o):
+        return isinstance(o, Index) and o.id == self.id and o.plev == self.plev
+
+    def __hash__(self):
+        return hash((self.id, self.plev))
+
+    def __repr__(self):
+        return "Index(%d|%s|%s%s)" % (self.dim, self.id, self.tags, "'" * self.plev)
+
+
+def _dtype_name(t):
+    return str(t.dtype).split(".")[-1]
+
+
+class ITensor:
+    """Dense tensor: a flat column-major buffer over `inds` (first fastest),
+    on the host (numpy) or on the device (torch tensor on the GPU)."""
+
+    def __init__(self, inds, data):
+        self.inds = tuple(inds)
+        self.data = data
+
+    # -------------------------------------------------------------- basics
+    @property
+    def dims(self):
+        return tuple(i.dim for i in self.inds)
+
+    @property
+    def ondevice(self):
+        return not isinstance(self.data, np.ndarray)
+
+    def order(self):
+        return len(self.inds)
+
+    def __repr__(self):
+        return "ITensor(%s, %s)" % (list(self.inds), "device" if self.ondevice else "host")
+
+    def _need_device(self):
+        if not self.ondevice:
+            raise _lib.ArgumentError("compute on ITensor storage needs roc(T) first")
+
+    # --------------------------------------------------------- conversion
+    def roc(self):
+        if self.ondevice:
+            return self
+        return ITensor(self.inds, ops.to_device(self.data, str(self.data.dtype)))
+
+    def cpu(self):
+        if not self.ondevice:
+            return self
+        return ITensor(self.inds, ops.to_host(self.data))
+
+    def array(self, *inds):
+        """Host array with axes in the order `inds` (default: own order)."""
+        h = self.cpu()
+        a = np.reshape(h.data, h.dims, order="F") if h.inds else np.reshape(h.data, ())
+        if inds:
+            a = np.transpose(a, [list(self.inds).index(i) for i in inds])
+        return a
+
+    # ----------------------------------------------------------- algebra
+    def permute(self, *inds):
+        self._need_device()
+        if len(inds) != len(self.inds) or set(inds) != set(self.inds):
+            raise _lib.DimensionMismatch("permute: index sets differ")
+        perm = [list(self.inds).index(i) for i in inds]
+        if perm == list(range(len(perm))):
+            return ITensor(self.inds, ops.copy(self.data))
+        B, _ = ops.permute(self.data, self.dims, perm)
+        return ITensor(inds, B)
+
+    def __mul__(self, other):
+        if isinstance(other, (int, float)):
+            self._need_device()
+            out = ops.copy(self.data)
+            ops.scal(out, float(other))
+            return ITensor(self.inds, out)
+        return contract(self, other)
+
+    __rmul__ = lambda self, x: self.__mul__(x)
+
+    def __truediv__(self, x):
+        return self * (1.0 / float(x))
+
+    def _aligned(self, other):
+        if set(self.inds) != set(other.inds) or len(self.inds) != len(other.inds):
+            raise _lib.DimensionMismatch("add: index sets differ")
+        return [list(other.inds).index(i) for i in self.inds]
+
+    def __add__(self, other):
+        self._need_device()
+        other._need_device()
+        perm = self._aligned(other)
+        out = ops.copy(self.data)
+        ops.add_permuted(other.data, other.dims, perm, 1.0, out, 1.0)
+        return ITensor(self.inds, out)
+
+    def __sub__(self, other):
+        self._need_device()
+        other._need_device()
+        perm = self._aligned(other)
+        out = ops.copy(self.data)
+        ops.add_permuted(other.data, other.dims, perm, -1.0, out, 1.0)
+        return ITensor(self.inds, out)
+
+    def norm(self):
+        self._need_device()
+        return ops.nrm2(self.data)
+
+    def scalar(self):
+        if self.inds:
+            raise _lib.DimensionMismatch("scalar of a tensor with indices")
+        return float(self.cpu().data[0])
+
+    def prime(self, n=1, inds=None):
+        sel = set(inds) if inds is not None else None
+        return ITensor([i.prime(n) if sel is None or i in sel else i for i in self.inds], self.data)
+
+    def noprime(self):
+        return ITensor([i.noprime() for i in self.inds], self.data)
+
+    def dag(self):
+        return self   # real storage
+
+
+# ---------------------------------------------------------------- factories
+def itensor(array, *inds, dtype="float64"):
+    """Host ITensor from an array whose axes follow `inds`."""
+    a = np.asarray(array, dtype=dtype)
+    if a.size != int(np.prod([i.dim for i in inds])):
+        raise _lib.DimensionMismatch("array size does not match the indices")
+    return ITensor(inds, np.ravel(np.reshape(a, [i.dim for i in inds], order="F"), order="F"))
+
+
+def roc_itensor(value_or_array, *inds, dtype="float64"):
+    """cuITensor analogue: device ITensor from a scalar fill or an array
+    (src/cuitensor.jl:1-26)."""
+    n = int(np.prod([i.dim for i in inds])) if inds else 1
+    if np.isscalar(value_or_array):
+        t = ops.empty(n, dtype)
+        ops.fill(t, float(value_or_array))
+        return ITensor(inds, t)
+    return itensor(value_or_array, *inds, dtype=dtype).roc()
+
+
+def random_roc_itensor(*inds, seed=0, offset=0, dtype="float64"):
+    """randomCuITensor analogue: Gaussian entries from the Philox stream."""
+    n = int(np.prod([i.dim for i in inds])) if inds else 1
+    return ITensor(inds, ops.randn(n, seed, offset, dtype))
+
+
+def roc(x):
+    return x.roc()
+
+
+def cpu(x):
+    return x.cpu()
+
+
+# ------------------------------------------------------------- contraction
+def contract(A, B, mode="native"):
+    """A*B: common indices summed; result indices = A's unique then B's
+    unique (NDTensors contract_labels; contract!! dispatch of
+    src/tensor/cudense.jl:83-110, one GETT kernel for every branch)."""
+    A._need_device()
+    B._need_device()
+    labels = {}
+    for i in list(A.inds) + list(B.inds):
+        labels.setdefault(i, len(labels) + 1)
+    la = [labels[i] for i in A.inds]
+    lb = [labels[i] for i in B.inds]
+    out_inds = [i for i in A.inds if i not in B.inds] + [i for i in B.inds if i not in A.inds]
+    for i in A.inds:
+        if i in B.inds and B.inds[B.inds.index(i)].dim != i.dim:
+            raise _lib.DimensionMismatch("contract: index dimension mismatch")
+    C, _ = ops.contract(A.data, A.dims, la, B.data, B.dims, lb, [labels[i] for i in out_inds],
+                        mode=mode)
+    return ITensor(out_inds, C)
+
+
+# ------------------------------------------------------------ factorisations
+def _matricize(A, Linds):
+    Linds = [i for i in A.inds if i in set(Linds)]
+    Rinds = [i for i in A.inds if i not in set(Linds)]
+    if not Linds or not Rinds:
+        raise _lib.DimensionMismatch("factorisation needs left and right indices")
+    P = A.permute(*(Linds + Rinds)) if list(A.inds) != Linds + Rinds else ITensor(A.inds, ops.copy(A.data))
+    m = int(np.prod([i.dim for i in Linds]))
+    n = int(np.prod([i.dim for i in Rinds]))
+    return P, Linds, Rinds, m, n
+
+
+class Spectrum:
+    def __init__(self, eigs, truncerr):
+        self.eigs = eigs
+        self.truncerr = truncerr
+
+
+def svd(A, Linds, maxdim=None, mindim=1, cutoff=0.0, absolute_cutoff=False,
+        relative_cutoff=True, lefttags="Link,u", righttags="Link,v"):
+    """U, S, V, spec with A ~ U*S*V (culinearalgebra.jl:33-72); S is a dense
+    k x k diagonal ITensor as in the reference."""
+    P, Li, Ri, m, n = _matricize(A, Linds)
+    U, S, V, k, te, _ = ops.svd_trunc(P.data, m, n, maxdim=maxdim, mindim=mindim, cutoff=cutoff,
+                                      absolute_cutoff=absolute_cutoff,
+                                      relative_cutoff=relative_cutoff)
+    u, v = Index(k, lefttags), Index(k, righttags)
+    s = ops.to_host(S)[:k]
+    dt = _dtype_name(A.data)
+    Ut = ITensor(Li + [u], U[: m * k])
+    Vt = ITensor(Ri + [v], V[: n * k])
+    St = itensor(np.diag(s), u, v, dtype=dt).roc()
+    return Ut, St, Vt, Spectrum(s ** 2, te)
+
+
+def eigen(A, Linds, Rinds, maxdim=None, cutoff=None, absolute_cutoff=False,
+          relative_cutoff=True, tags="Link,eigen"):
+    """Hermitian eigen: D (k), U (Rinds', r) with A ~ U D U^T, descending
+    (culinearalgebra.jl:74-108). Returns (D, U, spec)."""
+    P, Li, Ri, m, n = _matricize(A, Linds)
+    if m != n:
+        raise _lib.DimensionMismatch("eigen needs a square matricisation")
+    w, U, k, te, _ = ops.eigh_trunc(P.data, m, maxdim=maxdim,
+                                    cutoff=-1.0 if cutoff is None else cutoff,
+                                    absolute_cutoff=absolute_cutoff,
+                                    relative_cutoff=relative_cutoff)
+    r = Index(k, tags)
+    wh = ops.to_host(w)[:k]
+    return wh, ITensor(Li + [r], U[: m * k]), Spectrum(wh, te)
+
+
+def qr(A, Linds, tags="Link,qr"):
+    """Thin QR (culinearalgebra.jl:110-121)."""
+    P, Li, Ri, m, n = _matricize(A, Linds)
+    Q, R = ops.qr(P.data, m, n)
+    q = Index(min(m, n), tags)
+    return ITensor(Li + [q], Q), ITensor([q] + Ri, R)
+
+
+def factorize(A, Linds, ortho="left", which_decomp=None, maxdim=None, mindim=1, cutoff=None,
+              tags="Link,fact"):
+    """ITensors factorize: "eigen" (density matrix) when cutoff >= 1e-12,
+    else "svd"; returns (L, R, spec) with L or R orthonormal per `ortho`."""
+    if which_decomp is None:
+        which_decomp = "svd" if cutoff is None or cutoff < 1e-12 else "eigen"
+    Li = [i for i in A.inds if i in set(Linds)]
+    Ri = [i for i in A.inds if i not in set(Linds)]
+    if which_decomp == "svd":
+        U, S, V, spec = svd(A, Li, maxdim=maxdim, mindim=mindim,
+                            cutoff=0.0 if cutoff is None else cutoff, lefttags=tags,
+                            righttags=tags)
+        if ortho == "left":
+            return U, contract(S, V), spec
+        return contract(U, S), V, spec
+    if which_decomp == "eigen":
+        if ortho == "left":
+            Ap = A.prime(1, Li)
+            rho = contract(A, Ap)           # (Li', Li) with the right indices summed
+            D, U, spec = eigen(rho.noprime_pairs(Li), Li, [i.prime() for i in Li],
+                               maxdim=maxdim, cutoff=cutoff, tags=tags)
+            return U, contract(U, A), spec
+        Ap = A.prime(1, Ri)
+        rho = contract(A, Ap)
+        D, U, spec = eigen(rho.noprime_pairs(Ri), Ri, [i.prime() for i in Ri], maxdim=maxdim,
+                           cutoff=cutoff, tags=tags)
+        return contract(A, U), U, spec
+    raise _lib.ArgumentError("factorize: unknown which_decomp %r" % (which_decomp,))
+
+
+def _noprime_pairs(self, inds):
+    """rho(i..', i..) -> indices (i.., i..') with the unprimed set first, as
+    eigen(rho, Linds, Linds') expects."""
+    order = [i for i in self.inds if i in set(inds)] + [i for i in self.inds if i not in set(inds)]
+    return self.permute(*order)
+
+
+ITensor.noprime_pairs = _noprime_pairs

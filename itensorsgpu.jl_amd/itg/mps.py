@@ -1,0 +1,502 @@
+"""MPS/MPO on device storage, ProjMPO, davidson and two-site dmrg.
+
+Host orchestration in the shape of ITensors 0.2's `dmrg(H, psi0, sweeps)`
+(reached by the reference through test/dmrg.jl:27,75 and
+examples/dmrg.jl:25), with every FLOP in libitg:
+  - sweepnext: bonds 1..N-1 left to right then N-1..1 (2(N-1) per sweep)
+  - ProjMPO product: the fused two-site Heff apply (itg_heff2_apply)
+  - davidson: Krylov/Davidson on device vectors (FP64 dots, norms and
+    orthogonalisation), host-side projected eigenproblem, optional
+    diagonal or learned preconditioner with a revert guard
+  - replacebond!: factorize with "eigen" (density matrix) when
+    cutoff >= 1e-12 else "svd", then normalisation of the centre
+  - environments: L.A.W.A^dag / A.W.A^dag.R contractions
+Tensor index orders: MPS site (l, s, r) with dimension-1 boundary links;
+MPO site (wl, wr, s', s) with wl = 1 on the first and wr = 1 on the last site;
+environments L (a', w, a) and R (b', w, b).
+`mode` selects the arithmetic of the Heff and environment contractions
+("native" = FP64 MFMA, "fp32", "bf16x3"); storage, Davidson vector algebra,
+the density matrix, its eigendecomposition and the energy stay FP64.
+"""
+import math
+import time
+
+import numpy as np
+
+from . import _lib
+from . import ops
+from .itensor import ITensor, Index
+
+
+# ---------------------------------------------------------------- operators
+def spin_ops(S):
+    d = int(round(2 * S + 1))
+    m = S - np.arange(d)
+    Sz = np.diag(m)
+    Sp = np.zeros((d, d))
+    for i in range(1, d):
+        Sp[i - 1, i] = math.sqrt(S * (S + 1) - m[i] * (m[i] + 1))
+    Sm = Sp.T.copy()
+    return {"Sz": Sz, "S+": Sp, "S-": Sm, "Sx": 0.5 * (Sp + Sm), "Id": np.eye(d)}
+
+
+class OpSum:
+    """AutoMPO-style sum of one- and two-site terms (the `ampo` of
+    test/dmrg.jl:9-14,64-68), compiled to an MPO by a finite-state
+    construction: the channel (i, op) carries op_i from site i to every
+    partner j > i of a term (i, op, j, op'), whose coefficient and op' are
+    applied at j. Bond dimension at a cut = 2 + number of open channels
+    (5 for the Heisenberg chain)."""
+
+    def __init__(self):
+        self.one = []   # (coef, op, i)
+        self.two = []   # (coef, opA, i, opB, j)
+
+    def add(self, coef, op1, i, op2=None, j=None):
+        if op2 is None:
+            self.one.append((float(coef), op1, int(i)))
+        else:
+            i, j = int(i), int(j)
+            if i == j:
+                raise _lib.ArgumentError("two-site term on one site")
+            if i > j:
+                op1, op2, i, j = op2, op1, j, i
+            self.two.append((float(coef), op1, i, op2, j))
+        return self
+
+    def to_mpo_arrays(self, N, S=0.5):
+        ops_ = spin_ops(S)
+        d = ops_["Id"].shape[0]
+        # channels at cut k (between sites k and k+1)
+        chans = []
+        for k in range(N - 1):
+            cs = []
+            for (_, opA, i, _, j) in self.two:
+                if i <= k < j and (i, opA) not in cs:
+                    cs.append((i, opA))
+            chans.append(cs)
+        Ws = []
+        for k in range(N):
+            left = ["done"] + (chans[k - 1] if k > 0 else []) + ["start"] if k > 0 else ["start"]
+            right = ["done"] + (chans[k] if k < N - 1 else []) + ["start"] if k < N - 1 else ["done"]
+            W = np.zeros((len(left), len(right), d, d))
+            li = {c: a for a, c in enumerate(left)}
+            ri = {c: a for a, c in enumerate(right)}
+            I = ops_["Id"]
+            if "start" in li and "start" in ri:
+                W[li["start"], ri["start"]] += I
+            if "done" in li and "done" in ri:
+                W[li["done"], ri["done"]] += I
+            for (c, op, i) in self.one:
+                if i == k:
+                    W[li["start"], ri["done"]] += c * ops_[op]
+            for (c, opA, i, opB, j) in self.two:
+                if i == k:
+                    W[li["start"], ri[(i, opA)]] = ops_[opA]
+                elif j == k:
+                    W[li[(i, opA)], ri["done"]] += c * ops_[opB]
+            # channels passing through site k
+            for ch in (chans[k - 1] if k > 0 else []):
+                if k < N - 1 and ch in ri:
+                    W[li[ch], ri[ch]] = I
+            Ws.append(W)
+        return Ws
+
+
+def heisenberg_opsum(bonds, J=1.0):
+    os_ = OpSum()
+    for (i, j) in bonds:
+        os_.add(J, "Sz", i, "Sz", j)
+        os_.add(0.5 * J, "S+", i, "S-", j)
+        os_.add(0.5 * J, "S-", i, "S+", j)
+    return os_
+
+
+def tfim_opsum(N, J=1.0, h=0.5):
+    os_ = OpSum()
+    for j in range(N):
+        if j < N - 1:
+            os_.add(-J, "Sz", j, "Sz", j + 1)
+        os_.add(-h, "Sx", j)
+    return os_
+
+
+def cylinder_bonds(Lx, Ly, snake=True):
+    def site(x, y):
+        if snake and x % 2 == 1:
+            y = Ly - 1 - y
+        return x * Ly + y
+
+    bonds = set()
+    for x in range(Lx):
+        for y in range(Ly):
+            s = site(x, y)
+            if x + 1 < Lx:
+                bonds.add(tuple(sorted((s, site(x + 1, y)))))
+            t = site(x, (y + 1) % Ly)
+            if t != s:
+                bonds.add(tuple(sorted((s, t))))
+    return sorted(bonds)
+
+
+# ------------------------------------------------------------- containers
+class MPO:
+    """Device MPO: per site a flat buffer with dims (wl, wr, d, d)."""
+
+    def __init__(self, arrays, dtype="float64"):
+        self.dims = [tuple(int(x) for x in W.shape) for W in arrays]
+        self.data = [ops.to_device(np.ravel(W, order="F"), dtype) for W in arrays]
+
+    def __len__(self):
+        return len(self.data)
+
+    @staticmethod
+    def from_opsum(os_, N, S=0.5, dtype="float64"):
+        return MPO(os_.to_mpo_arrays(N, S), dtype)
+
+
+class MPS:
+    """Device MPS: per site a flat buffer with dims (l, s, r)."""
+
+    def __init__(self, data, dims, llim=0, rlim=None):
+        self.data = list(data)
+        self.dims = [tuple(x) for x in dims]
+        self.llim = llim
+        self.rlim = len(data) + 1 if rlim is None else rlim
+
+    def __len__(self):
+        return len(self.data)
+
+    def maxlinkdim(self):
+        return max(dm[2] for dm in self.dims)
+
+    def to_host(self):
+        return [np.reshape(ops.to_host(t), dm, order="F") for t, dm in zip(self.data, self.dims)]
+
+    @staticmethod
+    def from_host(arrays, dtype="float64"):
+        return MPS([ops.to_device(np.ravel(a, order="F"), dtype) for a in arrays],
+                   [a.shape for a in arrays])
+
+    def itensors(self, sites=None):
+        """ITensor view with fresh link indices (and given site indices)."""
+        N = len(self)
+        links = [Index(self.dims[0][0], "Link,l=0")] + \
+                [Index(self.dims[j][2], "Link,l=%d" % (j + 1)) for j in range(N)]
+        sites = sites or [Index(dm[1], "Site,n=%d" % j) for j, dm in enumerate(self.dims)]
+        return [ITensor((links[j], sites[j], links[j + 1]), self.data[j]) for j in range(N)]
+
+
+def link_dims(N, d, chi):
+    return [1] + [int(min(chi, d ** min(j, N - j))) for j in range(1, N)] + [1]
+
+
+def random_mps(N, d, chi, seed, dtype="float64"):
+    """Gaussian site tensors drawn in site order from one Philox stream
+    (itg_randn), right-orthogonalised by device QR and normalised: the
+    orthogonality centre is site 0 (randomCuMPS + orthogonalize!, cumps.jl:44-53)."""
+    dims = link_dims(N, d, chi)
+    data, shp, off = [], [], 0
+    for j in range(N):
+        n = dims[j] * d * dims[j + 1]
+        data.append(ops.randn(n, seed, off, dtype))
+        shp.append((dims[j], d, dims[j + 1]))
+        off += n
+    psi = MPS(data, shp)
+    right_orthogonalize(psi)
+    nrm = ops.nrm2(psi.data[0])
+    ops.scal(psi.data[0], 1.0 / nrm)
+    return psi
+
+
+def right_orthogonalize(psi):
+    """orthogonalize!(psi, 1): LQ of every site from the right end."""
+    for j in range(len(psi) - 1, 0, -1):
+        dl, d, dr = psi.dims[j]
+        Mt, _ = ops.permute(psi.data[j], (dl, d * dr), (1, 0))       # (d dr) x dl
+        Q, R = ops.qr(Mt, d * dr, dl)                                 # Q: (d dr) x k
+        k = min(d * dr, dl)
+        A, _ = ops.permute(Q, (d * dr, k), (1, 0))                    # k x (d dr)
+        psi.data[j] = A
+        psi.dims[j] = (k, d, dr)
+        pl, pd, _ = psi.dims[j - 1]
+        out = ops.empty(pl * pd * k, str(A.dtype).split(".")[-1])
+        ops.gemm("N", "T", pl * pd, k, dl, 1.0, psi.data[j - 1], pl * pd, R, k, 0.0, out, pl * pd)
+        psi.data[j - 1] = out
+        psi.dims[j - 1] = (pl, pd, k)
+    psi.llim, psi.rlim = 0, 2
+
+
+# ------------------------------------------------------------ environments
+def ones_env(dtype="float64"):
+    t = ops.empty(1, dtype)
+    ops.fill(t, 1.0)
+    return t, (1, 1, 1)
+
+
+def update_left(L, Ld, A, Ad, W, Wd, mode="native"):
+    """L_new(r', wr, r) = L(l', wl, l) A(l, s, r) W(wl, wr, s', s) A(l', s', r')."""
+    T1, d1 = ops.contract(L, Ld, [1, 2, 3], A, Ad, [3, 4, 5], [1, 2, 4, 5], mode=mode)
+    T2, d2 = ops.contract(T1, d1, [1, 2, 4, 5], W, Wd, [2, 6, 7, 4], [1, 7, 5, 6])
+    return ops.contract(T2, d2, [1, 7, 5, 6], A, Ad, [1, 7, 8], [8, 6, 5], mode=mode)
+
+
+def update_right(R, Rd, A, Ad, W, Wd, mode="native"):
+    """R_new(l', wl, l) = R(r', wr, r) A(l, s, r) W(wl, wr, s', s) A(l', s', r')."""
+    T1, d1 = ops.contract(R, Rd, [1, 2, 3], A, Ad, [4, 5, 3], [1, 2, 4, 5], mode=mode)
+    T2, d2 = ops.contract(T1, d1, [1, 2, 4, 5], W, Wd, [6, 2, 7, 5], [1, 7, 4, 6])
+    return ops.contract(T2, d2, [1, 7, 4, 6], A, Ad, [8, 7, 1], [8, 6, 4], mode=mode)
+
+
+class ProjMPO:
+    """Left/right environments of H around a two-site window."""
+
+    def __init__(self, H, mode="native"):
+        self.H = H
+        self.mode = mode
+        N = len(H)
+        self.L = [None] * (N + 1)
+        self.R = [None] * (N + 1)
+
+    def build_right(self, psi, upto=2):
+        N = len(psi)
+        self.R[N] = ones_env()
+        for j in range(N - 1, upto - 1, -1):
+            R, Rd = self.R[j + 1]
+            self.R[j] = update_right(R, Rd, psi.data[j], psi.dims[j], self.H.data[j],
+                                     self.H.dims[j], self.mode)
+        self.L[0] = ones_env()
+
+    def product(self, b, x, xdims, out=None, work=None):
+        L, Ld = self.L[b]
+        R, Rd = self.R[b + 2]
+        wl, wm = self.H.dims[b][0], self.H.dims[b][1]
+        wr = self.H.dims[b + 1][1]
+        chiL, d, _, chiR = xdims
+        return ops.heff2_apply(L, self.H.data[b], self.H.data[b + 1], R, x, chiL, wl, wm, wr, d,
+                               chiR, mode=self.mode, out=out, work=work)
+
+    def diag(self, b, xdims):
+        L, Ld = self.L[b]
+        R, Rd = self.R[b + 2]
+        wl, wm = self.H.dims[b][0], self.H.dims[b][1]
+        wr = self.H.dims[b + 1][1]
+        chiL, d, _, chiR = xdims
+        return ops.heff2_diag(L, self.H.data[b], self.H.data[b + 1], R, chiL, wl, wm, wr, d, chiR)
+
+
+# ----------------------------------------------------------------- davidson
+def davidson(matvec, x0, maxiter=2, miniter=1, errgoal=1e-14, northo=2, precond=None,
+             stats=None):
+    """ITensors-style Davidson on device vectors. Returns (lambda, phi).
+    precond(q, lam) -> correction (device) or None; a preconditioned step
+    whose Ritz value does not decrease the residual norm is reverted to the
+    plain (unpreconditioned) direction."""
+    nrm = ops.nrm2(x0)
+    phi = ops.copy(x0)
+    ops.scal(phi, 1.0 / nrm)
+    V = [phi]
+    AV = [matvec(phi)]
+    lam = ops.dot(V[0], AV[0])
+    q = ops.copy(AV[0])
+    ops.axpby(-lam, V[0], 1.0, q)
+    M = np.array([[lam]])
+    last = float("nan")
+    maxiter = min(maxiter, phi.numel() - 1)
+    nmv = 1
+    for ni in range(1, maxiter + 1):
+        qn = ops.nrm2(q)
+        conv = (qn < errgoal and abs(lam - last) < errgoal) or qn < max(1e-12, errgoal * 1e-3)
+        if qn < 1e-20 or (conv and ni > miniter):
+            break
+        last = lam
+        t = None
+        if precond is not None:
+            t = precond(q, lam)
+        if t is None:
+            t = ops.copy(q)
+        for _ in range(northo):
+            for v in V:
+                ops.axpby(-ops.dot(v, t), v, 1.0, t)
+        tn = ops.nrm2(t)
+        if tn < 1e-14 * max(1.0, qn):
+            break
+        ops.scal(t, 1.0 / tn)
+        V.append(t)
+        AV.append(matvec(t))
+        nmv += 1
+        m = len(V)
+        Mn = np.zeros((m, m))
+        Mn[: m - 1, : m - 1] = M
+        for k in range(m):
+            Mn[k, m - 1] = Mn[m - 1, k] = ops.dot(V[k], AV[m - 1])
+        M = Mn
+        ev, U = np.linalg.eigh(M)
+        lam = float(ev[0])
+        u = U[:, 0]
+        if u[0] < 0:
+            u = -u
+        phi = ops.copy(V[0])
+        ops.scal(phi, u[0])
+        q = ops.copy(AV[0])
+        ops.scal(q, u[0])
+        for k in range(1, m):
+            ops.axpby(u[k], V[k], 1.0, phi)
+            ops.axpby(u[k], AV[k], 1.0, q)
+        ops.axpby(-lam, phi, 1.0, q)
+    ops.scal(phi, 1.0 / ops.nrm2(phi))
+    if stats is not None:
+        stats["matvecs"] = stats.get("matvecs", 0) + nmv
+    return lam, phi
+
+
+def diag_preconditioner(dg):
+    """Davidson correction t = q / (diag(H) - lambda), regularised."""
+
+    def pre(q, lam):
+        return ops.precond_diag(q, dg, lam, 1e-8)
+
+    return pre
+
+
+# --------------------------------------------------------------- factorize
+def replacebond(psi, b, phi, pdims, ortho, maxdim, mindim, cutoff):
+    """replacebond!(psi, b, phi; ortho, normalize=true) via factorize."""
+    l, d1, d2, r = pdims
+    m, n = l * d1, d2 * r
+    use_eigen = cutoff is not None and cutoff >= 1e-12
+    dt = str(phi.dtype).split(".")[-1]
+    if use_eigen:
+        if ortho == "left":
+            rho = ops.empty(m * m, dt)
+            ops.gemm("N", "T", m, m, n, 1.0, phi, m, phi, m, 0.0, rho, m)
+            w, U, k, terr, _ = ops.eigh_trunc(rho, m, maxdim=maxdim, cutoff=cutoff)
+            A = U[: m * k]
+            B = ops.empty(k * n, dt)
+            ops.gemm("T", "N", k, n, m, 1.0, U, m, phi, m, 0.0, B, k)
+            ops.scal(B, 1.0 / ops.nrm2(B))
+        else:
+            rho = ops.empty(n * n, dt)
+            ops.gemm("T", "N", n, n, m, 1.0, phi, m, phi, m, 0.0, rho, n)
+            w, U, k, terr, _ = ops.eigh_trunc(rho, n, maxdim=maxdim, cutoff=cutoff)
+            B, _ = ops.permute(U[: n * k], (n, k), (1, 0))
+            A = ops.empty(m * k, dt)
+            ops.gemm("N", "N", m, k, n, 1.0, phi, m, U, n, 0.0, A, m)
+            ops.scal(A, 1.0 / ops.nrm2(A))
+    else:
+        work = ops.copy(phi)
+        U, S, V, k, terr, _ = ops.svd_trunc(work, m, n, maxdim=maxdim, mindim=mindim,
+                                            cutoff=0.0 if cutoff is None else cutoff)
+        s = ops.to_host(S)[:k]
+        Sd = ops.to_device(np.ravel(np.diag(s), order="F"), dt)
+        if ortho == "left":
+            A = U[: m * k]
+            B = ops.empty(k * n, dt)
+            ops.gemm("N", "T", k, n, k, 1.0, Sd, k, V, n, 0.0, B, k)
+            ops.scal(B, 1.0 / ops.nrm2(B))
+        else:
+            B, _ = ops.permute(V[: n * k], (n, k), (1, 0))
+            A = ops.empty(m * k, dt)
+            ops.gemm("N", "N", m, k, k, 1.0, U, m, Sd, k, 0.0, A, m)
+            ops.scal(A, 1.0 / ops.nrm2(A))
+    psi.data[b], psi.dims[b] = A, (l, d1, k)
+    psi.data[b + 1], psi.dims[b + 1] = B, (k, d2, r)
+    if ortho == "left":
+        psi.llim, psi.rlim = b + 1, b + 3
+    else:
+        psi.llim, psi.rlim = b - 1, b + 1
+    return terr, k
+
+
+# --------------------------------------------------------------------- dmrg
+class Sweeps:
+    """Sweep schedule (Sweeps/maxdim!/mindim!/cutoff!/noise!, test/dmrg.jl:19-24);
+    the last given value repeats."""
+
+    def __init__(self, nsweep, maxdim=(), mindim=(1,), cutoff=(0.0,), noise=(0.0,)):
+        self.nsweep = nsweep
+
+        def fill(v):
+            v = list(v) if v else [None]
+            return [v[min(i, len(v) - 1)] for i in range(nsweep)]
+
+        self.maxdim = fill(maxdim)
+        self.mindim = fill(mindim)
+        self.cutoff = fill(cutoff)
+        self.noise = fill(noise)
+
+
+def sweepnext(N):
+    for b in range(N - 1):
+        yield b, 1
+    for b in range(N - 2, -1, -1):
+        yield b, 2
+
+
+def dmrg(H, psi, sweeps, mode="native", krylovdim=3, precond=None, errgoal=1e-14,
+         record=None, sweep_callback=None, env_mode=None):
+    """Two-site DMRG; psi (right-orthogonal, centre at site 0) is updated in
+    place. Returns the final energy. `precond`: None, "diag" or a callable
+    factory (PH, b, xdims) -> precond(q, lam)."""
+    N = len(psi)
+    env_mode = mode if env_mode is None else env_mode
+    PH = ProjMPO(H, env_mode)
+    PH.build_right(psi)
+    energy = 0.0
+    for sw in range(sweeps.nsweep):
+        t0 = time.perf_counter()
+        maxerr, stats = 0.0, {}
+        if sweeps.noise[sw]:
+            # the density-matrix noise term of ITensors is not built; run
+            # without it (noise 1e-10..1e-11 in the reference tests)
+            stats["noise_ignored"] = sweeps.noise[sw]
+        for b, ha in sweepnext(N):
+            A, Ad = psi.data[b], psi.dims[b]
+            B, Bd = psi.data[b + 1], psi.dims[b + 1]
+            phi, pdims = ops.contract(A, Ad, [1, 2, 3], B, Bd, [3, 4, 5], [1, 2, 4, 5])
+            work = ops.heff2_work(pdims[0], H.dims[b][0], H.dims[b][1], H.dims[b + 1][1],
+                                  pdims[1], pdims[3], str(phi.dtype).split(".")[-1])
+
+            def mv(x, b=b, pdims=pdims, work=work):
+                return PH.product(b, x, pdims, work=work)
+
+            pre = None
+            if precond == "diag":
+                pre = diag_preconditioner(PH.diag(b, pdims))
+            elif callable(precond):
+                pre = precond(PH, b, pdims)
+            energy, phi = davidson(mv, phi, maxiter=krylovdim - 1, errgoal=errgoal,
+                                   precond=pre, stats=stats)
+            ortho = "left" if ha == 1 else "right"
+            terr, k = replacebond(psi, b, phi, pdims, ortho, sweeps.maxdim[sw],
+                                  sweeps.mindim[sw], sweeps.cutoff[sw])
+            maxerr = max(maxerr, terr)
+            if ha == 1:
+                L, Ld = PH.L[b]
+                PH.L[b + 1] = update_left(L, Ld, psi.data[b], psi.dims[b], H.data[b], H.dims[b],
+                                          env_mode)
+            else:
+                R, Rd = PH.R[b + 2]
+                PH.R[b + 1] = update_right(R, Rd, psi.data[b + 1], psi.dims[b + 1],
+                                           H.data[b + 1], H.dims[b + 1], env_mode)
+        _lib.ctx().sync()
+        rec = {"sweep": sw + 1, "energy": energy, "maxerr": maxerr,
+               "maxlinkdim": psi.maxlinkdim(), "seconds": time.perf_counter() - t0,
+               "matvecs": stats.get("matvecs", 0)}
+        if record is not None:
+            record.append(rec)
+        if sweep_callback is not None:
+            sweep_callback(rec)
+    return energy
+
+
+def expect_energy(H, psi):
+    """<psi|H|psi>/<psi|psi> by a full left-to-right transfer on the device."""
+    L, Ld = ones_env()
+    Nn, Nd = ones_env()
+    for j in range(len(psi)):
+        L, Ld = update_left(L, Ld, psi.data[j], psi.dims[j], H.data[j], H.dims[j])
+        # norm transfer: N(r', r) = N(l', l) A(l, s, r) A(l', s, r')
+        T, td = ops.contract(Nn, Nd, [1, 2, 3], psi.data[j], psi.dims[j], [3, 4, 5], [1, 2, 4, 5])
+        Nn, Nd = ops.contract(T, td, [1, 2, 4, 5], psi.data[j], psi.dims[j], [1, 4, 6], [6, 2, 5])
+    return ops.to_host(L)[0] / ops.to_host(Nn)[0]

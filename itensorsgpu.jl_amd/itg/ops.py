@@ -93,6 +93,15 @@ def axpby(alpha, x, beta, y):
     return y
 
 
+def precond_diag(q, diag, lam, floor=1e-8):
+    """t = q ./ (diag - lam) (floored), the Davidson diagonal correction."""
+    t = empty(q.numel(), str(q.dtype).split(".")[-1])
+    c = ctx()
+    c.check(lib().itg_precond_diag(c.ptr, dtype_code(q), q.numel(), ptr(q), ptr(diag), float(lam),
+                                   float(floor), ptr(t)))
+    return t
+
+
 def dot(x, y):
     c = ctx()
     out = C.c_double()
@@ -253,4 +262,13 @@ def heff2_diag(L, W1, W2, R, chiL, wl, wm, wr, d, chiR):
     c = ctx()
     c.check(lib().itg_heff2_diag(c.ptr, dtype_code(L), chiL, wl, wm, wr, d, chiR, ptr(L),
                                  ptr(W1), ptr(W2), ptr(R), ptr(out)))
+    return out
+
+
+def copy(t):
+    """Device-to-device copy of a buffer (stream-ordered)."""
+    out = empty(t.numel(), str(t.dtype).split(".")[-1])
+    if t.numel():
+        c = ctx()
+        c.check(lib().itg_memcpy_d2d(c.ptr, ptr(out), ptr(t), t.numel() * t.element_size()))
     return out

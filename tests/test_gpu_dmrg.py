@@ -1,0 +1,187 @@
+"""GPU parity of the ITensor-level mirror, random MPS, Davidson and two-site
+DMRG against the CPU oracle, exact diagonalisation and the reference's own
+physics tests (test/dmrg.jl)."""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import dmrg as odmrg
+from oracle import models
+from oracle import tensor_ops as ops
+
+pytestmark = pytest.mark.gpu
+
+E0_N20 = -8.682473334398939
+
+
+@pytest.fixture(scope="module")
+def itg():
+    import itg as m
+    from itg import itensor, mps
+    m._lib.ctx()
+    return m
+
+
+# ------------------------------------------------------------ ITensor layer
+def test_itensor_roundtrip_and_permute(itg):
+    """test/test_cuitensor.jl:17-43 and test_cudense.jl:43-48."""
+    from itg.itensor import Index, itensor
+    i, j, k = Index(2, "i"), Index(3, "j"), Index(4, "k")
+    rng = np.random.default_rng(0)
+    a = rng.standard_normal((2, 4, 3))
+    T = itensor(a, i, k, j)
+    D = T.roc()
+    assert np.array_equal(D.cpu().data, T.data)
+    P = D.permute(k, j, i)
+    assert P.inds == (k, j, i)
+    assert np.array_equal(P.array(), np.transpose(a, (1, 2, 0)))
+    assert np.array_equal(P.array(i, k, j), a)
+
+
+def test_itensor_contract_add_norm(itg):
+    from itg.itensor import Index, itensor, random_roc_itensor
+    i, j, k, l = Index(2, "i"), Index(3, "j"), Index(4, "k"), Index(5, "l")
+    A = random_roc_itensor(i, j, k, seed=1)
+    B = random_roc_itensor(k, l, j, seed=2)
+    C = A * B
+    assert set(C.inds) == {i, l} and C.inds == (i, l)
+    want = np.einsum("ijk,klj->il", A.array(), B.array())
+    np.testing.assert_allclose(C.array(), want, rtol=1e-13)
+    # add with permutation (test_cuitensor.jl:89-99)
+    A2 = random_roc_itensor(k, i, j, seed=3)
+    S = A + A2
+    np.testing.assert_allclose(S.array(i, j, k), A.array(i, j, k) + A2.array(i, j, k), rtol=0,
+                               atol=1e-15)
+    D = A - A2
+    np.testing.assert_allclose(D.array(i, j, k), A.array(i, j, k) - A2.array(i, j, k), atol=1e-15)
+    # norm(A) = sqrt(scalar(A*A)) (test_cuitensor.jl:74-78)
+    assert math.isclose(A.norm(), math.sqrt((A * A).scalar()), rel_tol=1e-13)
+
+
+def test_itensor_factorizations(itg):
+    """test/test_cuitensor.jl:101-130 at the ITensor level."""
+    from itg.itensor import Index, qr, random_roc_itensor, svd, contract
+    i, j, k, l = Index(2, "i"), Index(3, "j"), Index(4, "k"), Index(5, "l")
+    A = random_roc_itensor(i, j, k, l, seed=4)
+    U, S, V, spec = svd(A, (j, l))
+    R = contract(contract(U, S), V)
+    np.testing.assert_allclose(R.array(i, j, k, l), A.array(i, j, k, l), atol=1e-13)
+    u = [x for x in U.inds if x not in A.inds][0]
+    UU = contract(U, U.prime(1, [u]))
+    np.testing.assert_allclose(UU.array(), np.eye(u.dim), atol=1e-14)
+    Q, Rr = qr(A, (i, l))
+    np.testing.assert_allclose(contract(Q, Rr).array(i, j, k, l), A.array(i, j, k, l), atol=1e-13)
+
+
+def test_contract_errors(itg):
+    from itg import DimensionMismatch
+    from itg.itensor import Index, random_roc_itensor
+    i, j = Index(2, "i"), Index(3, "j")
+    A = random_roc_itensor(i, j, seed=1)
+    B = random_roc_itensor(j, seed=2)
+    with pytest.raises(DimensionMismatch):
+        A + B
+
+
+# ----------------------------------------------------------------- davidson
+def test_davidson_10x10(itg):
+    """test/test_cuiterativesolvers.jl:13-27: M(v) ~ lambda v."""
+    from itg import ops as dev
+    from itg.mps import davidson
+    rng = np.random.default_rng(2)
+    A = rng.standard_normal((10, 10))
+    M = A @ A.T
+    dM = dev.to_device(ops.to_flat(M))
+
+    def mv(x):
+        y = dev.empty(10)
+        dev.gemm("N", "N", 10, 1, 10, 1.0, dM, 10, x, 10, 0.0, y, 10)
+        return y
+
+    lam, v = davidson(mv, dev.to_device(rng.standard_normal(10)), maxiter=10)
+    vh = dev.to_host(v)
+    np.testing.assert_allclose(M @ vh, lam * vh, atol=1e-9)
+    assert math.isclose(lam, np.linalg.eigvalsh(M)[0], rel_tol=1e-9)
+
+
+# -------------------------------------------------------------- random MPS
+def test_random_mps_matches_oracle(itg):
+    from itg.mps import random_mps
+    psi = random_mps(12, 2, 10, seed=0)
+    ref = odmrg.random_mps(12, 2, 10, seed=0)
+    for a, b in zip(psi.to_host(), ref):
+        assert a.shape == b.shape
+        np.testing.assert_allclose(a, b, atol=1e-12)
+
+
+# --------------------------------------------------------------------- dmrg
+def heis_mpo(N):
+    from itg.mps import MPO, heisenberg_opsum
+    return MPO.from_opsum(heisenberg_opsum([(j, j + 1) for j in range(N - 1)]), N)
+
+
+def test_dmrg_config1_n20_fp64(itg):
+    """S=1/2 Heisenberg N=20 (config 1): random MPS linkdims 10 seed 0,
+    5 sweeps maxdim [10,20,100,100,200], cutoff 1e-10, 2 Krylov vectors.
+    Compared with the oracle's identical run and with ED."""
+    from itg.mps import Sweeps, dmrg, expect_energy, random_mps
+    N = 20
+    H = heis_mpo(N)
+    psi = random_mps(N, 2, 10, seed=0)
+    rec = []
+    sw = Sweeps(5, maxdim=[10, 20, 100, 100, 200], cutoff=[1e-10])
+    E = dmrg(H, psi, sw, krylovdim=2, record=rec)
+    print(rec)
+    Eo = odmrg.dmrg(models.heisenberg_mpo(N), odmrg.random_mps(N, 2, 10, seed=0),
+                    maxdims=[10, 20, 100, 100, 200], cutoffs=[1e-10] * 5, krylovdim=2)
+    assert E0_N20 - 1e-10 <= E < E0_N20 + 1e-7
+    assert abs(E - Eo) < 1e-8, (E, Eo)
+    assert abs(expect_energy(H, psi) - E) < 1e-9
+
+
+def test_dmrg_tfim_reference(itg):
+    """test/dmrg.jl:58-81: critical TFIM N=32, relative error < 1e-2."""
+    from itg.mps import MPO, Sweeps, dmrg, random_mps, tfim_opsum
+    N = 32
+    H = MPO.from_opsum(tfim_opsum(N), N)
+    psi = random_mps(N, 2, 10, seed=432)
+    E = dmrg(H, psi, Sweeps(5, maxdim=[10, 20], cutoff=[1e-12], noise=[1e-10]))
+    Ex = models.tfim_exact_energy(N)
+    print(E, Ex)
+    assert abs((E - Ex) / Ex) < 1e-2
+    assert E >= Ex - 1e-9
+
+
+def test_dmrg_spin_one(itg):
+    """test/dmrg.jl:5-29: S=1 Heisenberg N=10, maxdim 10,20,40: E < -12."""
+    from itg.mps import MPO, Sweeps, dmrg, heisenberg_opsum, random_mps
+    N = 10
+    H = MPO.from_opsum(heisenberg_opsum([(j, j + 1) for j in range(N - 1)]), N, S=1.0)
+    psi = random_mps(N, 3, 10, seed=1)
+    E = dmrg(H, psi, Sweeps(3, maxdim=[10, 20, 40], mindim=[1, 10], cutoff=[1e-11]))
+    print(E)
+    assert E < -12.0
+
+
+@pytest.mark.parametrize("mode", ["fp32", "bf16x3"])
+def test_dmrg_mixed_precision_n20(itg, mode):
+    """Contractions in FP32 / split-BF16 with FP64 Davidson, density matrix
+    and energy: |dE|/N against the FP64 run."""
+    from itg.mps import Sweeps, dmrg, random_mps
+    N = 20
+    sw = Sweeps(6, maxdim=[10, 20, 100, 100, 200], cutoff=[1e-12])
+    E64 = dmrg(heis_mpo(N), random_mps(N, 2, 10, seed=0), sw)
+    Em = dmrg(heis_mpo(N), random_mps(N, 2, 10, seed=0), sw, mode=mode)
+    print(mode, E64, Em, abs(Em - E64) / N)
+    tol = {"fp32": 1e-7, "bf16x3": 1e-6}[mode]
+    assert abs(Em - E64) / N < tol
+
+
+def test_dmrg_diag_preconditioner(itg):
+    from itg.mps import Sweeps, dmrg, random_mps
+    N = 16
+    E = dmrg(heis_mpo(N), random_mps(N, 2, 10, seed=3),
+             Sweeps(5, maxdim=[10, 20, 64], cutoff=[1e-12]), precond="diag")
+    Eed = models.ground_energy(models.bond_hamiltonian(N, [(j, j + 1) for j in range(N - 1)]))
+    assert Eed - 1e-10 <= E < Eed + 1e-6
