@@ -136,7 +136,11 @@ class Context:
         if rc != ITG_OK:
             raise ITGError("itg_init failed with status %d" % rc)
         torch.cuda.set_device(device)
-        self.bind_stream(torch.cuda.current_stream(device).cuda_stream)
+        # a dedicated stream (default priority) shared by torch's allocator
+        # and libitg, so events recorded on it time the library's kernels
+        self.stream = torch.cuda.Stream(device=device)
+        torch.cuda.set_stream(self.stream)
+        self.bind_stream(self.stream.cuda_stream)
 
     def bind_stream(self, stream_handle):
         self.check(lib().itg_set_stream(self.ptr, _vp(stream_handle)))

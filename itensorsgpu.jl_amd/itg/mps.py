@@ -227,6 +227,46 @@ def right_orthogonalize(psi):
     psi.llim, psi.rlim = 0, 2
 
 
+# ------------------------------------------------------------------ timing
+class PhaseTimer:
+    """Accumulates device time per phase with events recorded on the stream
+    libitg runs on, plus the algorithmic FLOPs of each timed call."""
+
+    def __init__(self):
+        self.pairs = {}
+        self.flops = {}
+        self.calls = {}
+
+    def start(self):
+        import torch
+        ev = torch.cuda.Event(enable_timing=True)
+        ev.record()
+        return ev
+
+    def stop(self, name, ev0, flops=0.0):
+        import torch
+        ev1 = torch.cuda.Event(enable_timing=True)
+        ev1.record()
+        self.pairs.setdefault(name, []).append((ev0, ev1))
+        self.flops[name] = self.flops.get(name, 0.0) + flops
+        self.calls[name] = self.calls.get(name, 0) + 1
+
+    def report(self):
+        import torch
+        torch.cuda.synchronize()
+        out = {}
+        for name, prs in self.pairs.items():
+            ms = sum(a.elapsed_time(b) for a, b in prs)
+            out[name] = {"ms": ms, "calls": self.calls[name], "flops": self.flops[name]}
+        return out
+
+
+def heff_flops(chiL, wl, wr, d, chiR):
+    """Algorithmic FLOPs of one two-site Heff apply (C1 + W pass + C4)."""
+    return (2.0 * chiL * wl * d * d * chiR * chiL + 2.0 * chiL * chiR * (wl * d * d) * (d * d * wr)
+            + 2.0 * chiL * d * d * chiR * wr * chiR)
+
+
 # ------------------------------------------------------------ environments
 def ones_env(dtype="float64"):
     t = ops.empty(1, dtype)
@@ -251,9 +291,10 @@ def update_right(R, Rd, A, Ad, W, Wd, mode="native"):
 class ProjMPO:
     """Left/right environments of H around a two-site window."""
 
-    def __init__(self, H, mode="native"):
+    def __init__(self, H, mode="native", timer=None):
         self.H = H
         self.mode = mode
+        self.timer = timer
         N = len(H)
         self.L = [None] * (N + 1)
         self.R = [None] * (N + 1)
@@ -273,8 +314,12 @@ class ProjMPO:
         wl, wm = self.H.dims[b][0], self.H.dims[b][1]
         wr = self.H.dims[b + 1][1]
         chiL, d, _, chiR = xdims
-        return ops.heff2_apply(L, self.H.data[b], self.H.data[b + 1], R, x, chiL, wl, wm, wr, d,
-                               chiR, mode=self.mode, out=out, work=work)
+        ev = self.timer.start() if self.timer is not None else None
+        y = ops.heff2_apply(L, self.H.data[b], self.H.data[b + 1], R, x, chiL, wl, wm, wr, d,
+                            chiR, mode=self.mode, out=out, work=work)
+        if ev is not None:
+            self.timer.stop("heff", ev, heff_flops(chiL, wl, wr, d, chiR))
+        return y
 
     def diag(self, b, xdims):
         L, Ld = self.L[b]
@@ -434,14 +479,21 @@ def sweepnext(N):
 
 
 def dmrg(H, psi, sweeps, mode="native", krylovdim=3, precond=None, errgoal=1e-14,
-         record=None, sweep_callback=None, env_mode=None):
+         record=None, sweep_callback=None, env_mode=None, ph=None, timer=None):
     """Two-site DMRG; psi (right-orthogonal, centre at site 0) is updated in
     place. Returns the final energy. `precond`: None, "diag" or a callable
-    factory (PH, b, xdims) -> precond(q, lam)."""
+    factory (PH, b, xdims) -> precond(q, lam). Passing the ProjMPO `ph` of a
+    previous call continues from its environments (a completed sweep leaves
+    psi right-orthogonal and the right environments current)."""
     N = len(psi)
     env_mode = mode if env_mode is None else env_mode
-    PH = ProjMPO(H, env_mode)
-    PH.build_right(psi)
+    if ph is None:
+        PH = ProjMPO(H, env_mode, timer)
+        PH.build_right(psi)
+    else:
+        PH = ph
+        PH.mode, PH.timer = env_mode, timer
+    dmrg.last_projmpo = PH
     energy = 0.0
     for sw in range(sweeps.nsweep):
         t0 = time.perf_counter()
@@ -468,9 +520,13 @@ def dmrg(H, psi, sweeps, mode="native", krylovdim=3, precond=None, errgoal=1e-14
             energy, phi = davidson(mv, phi, maxiter=krylovdim - 1, errgoal=errgoal,
                                    precond=pre, stats=stats)
             ortho = "left" if ha == 1 else "right"
+            ev = timer.start() if timer is not None else None
             terr, k = replacebond(psi, b, phi, pdims, ortho, sweeps.maxdim[sw],
                                   sweeps.mindim[sw], sweeps.cutoff[sw])
+            if ev is not None:
+                timer.stop("factorize", ev)
             maxerr = max(maxerr, terr)
+            ev = timer.start() if timer is not None else None
             if ha == 1:
                 L, Ld = PH.L[b]
                 PH.L[b + 1] = update_left(L, Ld, psi.data[b], psi.dims[b], H.data[b], H.dims[b],
@@ -479,6 +535,8 @@ def dmrg(H, psi, sweeps, mode="native", krylovdim=3, precond=None, errgoal=1e-14
                 R, Rd = PH.R[b + 2]
                 PH.R[b + 1] = update_right(R, Rd, psi.data[b + 1], psi.dims[b + 1],
                                            H.data[b + 1], H.dims[b + 1], env_mode)
+            if ev is not None:
+                timer.stop("environment", ev)
         _lib.ctx().sync()
         rec = {"sweep": sw + 1, "energy": energy, "maxerr": maxerr,
                "maxlinkdim": psi.maxlinkdim(), "seconds": time.perf_counter() - t0,
