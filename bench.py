@@ -2,17 +2,21 @@
 """Two-site DMRG sweep throughput on one MI355X.
 
 Workload (the headline configuration): S=1/2 Heisenberg chain N=100, J=1,
-MPO bond dimension 5, random MPS (link dims 10, seed 1), maxdim ramp
-64,128,256,512,1024 (untimed set-up), then sweeps at maxdim 2048, cutoff
-1e-12, two-site Davidson with 3 Krylov vectors. A step is one full sweep
-(2(N-1) bond updates) at maxdim 2048. Both forms run from the same start:
-contractions in split-BF16 (3 products, FP32 accumulation; the headline
-`value`) and in FP64; Davidson vectors, density matrices, eigensolver and
-energies are FP64 in both. The Heff apply is timed live with events on the
-library's stream for the roofline; the CPU baseline is the numpy oracle on a
-bounded sample (one full-size bond update), scaled to a sweep.
+MPO bond dimension 5, two-site Davidson with 3 Krylov vectors, sweeps at
+maxdim 2048. A step is one full sweep (2(N-1) bond updates). Default start:
+a random right-canonical MPS with link dims min(2^j, 2^(N-j), 2048) (seed 1)
+and cutoff 0, so every bulk bond update runs at the full chi=2048 shapes
+(Heff on 2048x2x2x2048, eigensolver n=4096). --config3 instead starts from
+link dims 10 with the maxdim ramp 64..1024 and cutoff 1e-12 (with which the
+bond dimension stays far below 2048 in the first sweeps). Both forms run
+from the same start: contractions in split-BF16 (3 products, FP32
+accumulation; the headline `value`) and in FP64; Davidson vectors, density
+matrices, eigensolver and energies are FP64 in both. The Heff apply is timed
+live with events on the library's stream for the roofline; the CPU baseline
+is the numpy oracle on a bounded sample (one bond update at chi=1024),
+scaled to a sweep by the chi^3 cost model.
 
-  python bench.py [--steps K] [--warmup W] [--N 100] [--chi 2048]
+  python bench.py [--steps K] [--warmup W] [--N 100] [--chi 2048] [--config3]
   python bench.py --micro      # Heff apply of the chi=1024 microbench only
 """
 import argparse
@@ -40,27 +44,43 @@ def ramp_for(chi):
     return [m for m in (64, 128, 256, 512, 1024) if m < chi]
 
 
-def run_mode(mode, N, chi, steps, warmup, krylov):
+def run_mode(mode, N, chi, steps, warmup, krylov, config3=False):
+    """Sweeps at maxdim chi. Default start: a random right-canonical MPS with
+    link dims min(2^j, 2^(N-j), chi) (seed 1), cutoff 0, so every bulk bond
+    runs at the full chi x chi shape. config3: random MPS with link dims 10
+    and the ramp 64..1024 before the chi sweeps, cutoff 1e-12."""
     from itg import _lib
     from itg.mps import MPO, PhaseTimer, Sweeps, dmrg, heisenberg_opsum, random_mps
     import torch
     H = MPO.from_opsum(heisenberg_opsum([(j, j + 1) for j in range(N - 1)]), N)
-    psi = random_mps(N, 2, 10, seed=1)
-    ramp = ramp_for(chi)
     t0 = time.perf_counter()
-    dmrg(H, psi, Sweeps(len(ramp), maxdim=ramp, cutoff=[1e-12]), mode=mode, krylovdim=krylov,
-         sweep_callback=lambda r: log("[%s] ramp" % mode, r))
-    ph = dmrg.last_projmpo
+    cutoff = 1e-12 if config3 else 0.0
+    if config3:
+        psi = random_mps(N, 2, 10, seed=1)
+        ramp = ramp_for(chi)
+        dmrg(H, psi, Sweeps(len(ramp), maxdim=ramp, cutoff=[cutoff]), mode=mode,
+             krylovdim=krylov, sweep_callback=lambda r: log("[%s] ramp" % mode, r))
+        ph = dmrg.last_projmpo
+    else:
+        psi = random_mps(N, 2, chi, seed=1)
+        log("[%s] random MPS chi=%d ready (%.1f s)" % (mode, chi, time.perf_counter() - t0))
+        ph = None
     for _ in range(warmup):
-        dmrg(H, psi, Sweeps(1, maxdim=[chi], cutoff=[1e-12]), mode=mode, krylovdim=krylov, ph=ph,
-             sweep_callback=lambda r: log("[%s] warmup" % mode, r))
+        dmrg(H, psi, Sweeps(1, maxdim=[chi], cutoff=[cutoff]), mode=mode, krylovdim=krylov,
+             ph=ph, sweep_callback=lambda r: log("[%s] warmup" % mode, r))
+        ph = dmrg.last_projmpo
+    if ph is None:
+        from itg.mps import ProjMPO
+        ph = ProjMPO(H, mode)
+        ph.build_right(psi)
     setup_s = time.perf_counter() - t0
+    log("[%s] set-up done (%.1f s)" % (mode, setup_s))
     timer = PhaseTimer()
     rec = []
     torch.cuda.synchronize()
     _lib.ctx().sync()
     t1 = time.perf_counter()
-    E = dmrg(H, psi, Sweeps(steps, maxdim=[chi], cutoff=[1e-12]), mode=mode, krylovdim=krylov,
+    E = dmrg(H, psi, Sweeps(steps, maxdim=[chi], cutoff=[cutoff]), mode=mode, krylovdim=krylov,
              ph=ph, timer=timer, record=rec,
              sweep_callback=lambda r: log("[%s] timed" % mode, r))
     torch.cuda.synchronize()
@@ -85,9 +105,9 @@ def sweep_cost_units(N, d, chi, w):
 
 
 def cpu_baseline(chi, w=5, d=2, krylov=3, seed=0):
-    """The oracle (numpy FP64, BLAS/LAPACK) on one full-size bond update:
-    krylov Heff applies, the density matrix and its eigendecomposition, one
-    environment update; scaled by the sweep's bond-cost units."""
+    """The oracle (numpy FP64, BLAS/LAPACK) on one bond update at bond
+    dimension chi: krylov Heff applies, the density matrix and its
+    eigendecomposition, one environment update. Returns seconds."""
     import numpy as np
     from oracle import dmrg as odmrg
     from oracle import linalg as olin
@@ -99,11 +119,13 @@ def cpu_baseline(chi, w=5, d=2, krylov=3, seed=0):
     A = rng.standard_normal((chi, d, chi)) / chi
     t0 = time.perf_counter()
     x = psi
-    for _ in range(krylov):
+    for i in range(krylov):
         x = odmrg.heff_apply(L, W, W, R, x)
         x = x / np.linalg.norm(x)
+        log("cpu baseline: Heff apply %d done (%.1f s)" % (i + 1, time.perf_counter() - t0))
     P = np.reshape(x, (chi * d, d * chi), order="F")
     olin.eigh_trunc(P @ P.T, maxdim=chi, cutoff=1e-12)
+    log("cpu baseline: eigh n=%d done (%.1f s)" % (chi * d, time.perf_counter() - t0))
     odmrg.update_left(L, A, W)
     return time.perf_counter() - t0
 
@@ -152,12 +174,15 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--warmup", type=int, default=0)
     ap.add_argument("--N", type=int, default=100)
     ap.add_argument("--chi", type=int, default=2048)
     ap.add_argument("--krylov", type=int, default=3)
     ap.add_argument("--modes", default="bf16x3,native")
     ap.add_argument("--micro", action="store_true")
+    ap.add_argument("--config3", action="store_true",
+                    help="config-3 start: link dims 10, ramp 64..1024, cutoff 1e-12")
+    ap.add_argument("--cpu-chi", type=int, default=1024)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -174,7 +199,8 @@ def main():
     modes = args.modes.split(",")
     res = {}
     for m in modes:
-        res[m] = run_mode(m, args.N, args.chi, args.steps, args.warmup, args.krylov)
+        res[m] = run_mode(m, args.N, args.chi, args.steps, args.warmup, args.krylov,
+                          args.config3)
         log(m, {k: v for k, v in res[m].items() if k != "sweeps"})
     head = modes[0]
     r = res[head]
@@ -193,10 +219,14 @@ def main():
         "vs_baseline": None,
         "dtype": {"bf16x3": "bf16x3 (fp32 acc) + f64", "native": "f64", "fp32": "f32 + f64"}[head],
         "data": "synthetic",
-        "config": {"workload": "Heisenberg S=1/2 chain N=%d two-site DMRG, maxdim ramp %s then "
-                               "sweeps at maxdim %d, cutoff 1e-12, %d Krylov vectors, random MPS "
-                               "(link dims 10, seed 1); one step = one sweep at maxdim %d"
-                               % (args.N, ramp_for(args.chi), args.chi, args.krylov, args.chi),
+        "config": {"workload": ("Heisenberg S=1/2 chain N=%d two-site DMRG, %d Krylov vectors; "
+                                % (args.N, args.krylov)) +
+                               ("random MPS link dims 10 seed 1, maxdim ramp %s, cutoff 1e-12, "
+                                "then sweeps at maxdim %d" % (ramp_for(args.chi), args.chi)
+                                if args.config3 else
+                                "random right-canonical MPS at link dims min(2^j, 2^(N-j), %d) "
+                                "seed 1, cutoff 0 (bulk bonds at %d)" % (args.chi, args.chi)) +
+                               "; one step = one sweep at maxdim %d" % args.chi,
                    "N": args.N, "chi": args.chi, "mpo_bond_dim": 5, "modes": modes},
         "roofline": {"bound": "mfma", "kernel": "two-site Heff apply (GETT C1 + W pass + GETT C4)",
                      "achieved": achieved, "peak": PEAK[head], "unit": "TFLOP/s",
@@ -209,15 +239,18 @@ def main():
         line["dE_per_site"] = abs(e[0] - e[1]) / args.N
         line["speedup_vs_" + modes[1]] = res[modes[1]]["s_per_sweep"] / r["s_per_sweep"]
     if not args.no_cpu_baseline:
-        t_bond = cpu_baseline(args.chi)
-        units = sweep_cost_units(args.N, 2, args.chi, 5)
+        cchi = min(args.cpu_chi, args.chi)
+        t_bond = cpu_baseline(cchi)
+        # bond-cost units of a sweep relative to one chi=cchi bond (Heff and
+        # eigh both scale as chi^3 at fixed w, d)
+        units = sweep_cost_units(args.N, 2, args.chi, 5) * (args.chi / cchi) ** 3
         line["cpu_baseline"] = {
             "value": t_bond * units, "unit": "s/sweep", "cores": blas_threads(),
             "kind": "port",
             "sample": "oracle (numpy FP64, OpenBLAS/LAPACK) one bond update at chi=%d: %d Heff "
                       "applies + density-matrix eigh (n=%d) + one environment update = %.2f s, "
-                      "times %.1f full-bond units per sweep" % (args.chi, args.krylov,
-                                                               2 * args.chi, t_bond, units)}
+                      "times %.1f such bonds per sweep (chi^3 cost model)"
+                      % (cchi, args.krylov, 2 * cchi, t_bond, units)}
     print(json.dumps(line))
 
 
