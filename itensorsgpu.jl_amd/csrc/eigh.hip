@@ -4,11 +4,10 @@
 // sliced) used by replacebond! on the density matrix rho = phi phi^T.
 //
 // 1. Householder tridiagonalisation, blocked as in LAPACK dsytrd/dlatrd
-//    (panels of NB columns): per column a fused single-workgroup kernel
-//    (column update from the panel's V/W, reflector generation, finish of the
-//    previous w) and a symmetric matrix-vector product over the trailing
-//    block (one wavefront per column, coalesced, memory bound); per panel
-//    the rank-2*NB trailing update A -= V W^T + W V^T on FP64 MFMA.
+//    (panels of NB columns): per column two kernels spanning the GPU (the
+//    reflector fused into the symmetric matrix-vector product over the
+//    trailing block, then w and the next column's update; see below); per
+//    panel the rank-2*NB trailing update A -= V W^T + W V^T on FP64 MFMA.
 // 2. Divide and conquer on the tridiagonal (Cuppen tearing at every split,
 //    bottom-up merges of blocks 1,2,4,...): per merge a sort + deflation
 //    workgroup, rotation, secular-root, Gu-Eisenstat and eigenvector kernels,
@@ -50,123 +49,190 @@ __device__ double block_sum(double v, double* sh) {
 }
 
 // ------------------------------------------------------- tridiagonalisation
-__global__ __launch_bounds__(1024) void trd_step_kernel(double* __restrict__ A, int n, int p,
-                                                        int ifin, int icol,
-                                                        double* __restrict__ Vp,
-                                                        double* __restrict__ Wp,
-                                                        const double* __restrict__ y,
-                                                        double* d, double* e, double* tau) {
-  __shared__ double red[16];
-  __shared__ double xs[2 * NB];
-  __shared__ double sc[4];
-  const int t = threadIdx.x, nt = blockDim.x;
-  const size_t ld = (size_t)n;
-  // finish w of panel column ifin: w = tau (y - W x1 - V x2), w -= tau/2 (w.v) v
-  if (ifin >= 0) {
-    int jp = p + ifin;
-    double* w = Wp + (size_t)ifin * ld;
-    double tj = (jp < n - 1) ? tau[jp] : 0.0;
-    if (tj != 0.0) {
-      const double* v = Vp + (size_t)ifin * ld;
-      int lane = t & 63, wv = t >> 6, nw = nt >> 6;
-      for (int q = wv; q < 2 * ifin; q += nw) {
-        const double* col = (q < ifin) ? Vp + (size_t)q * ld : Wp + (size_t)(q - ifin) * ld;
-        double s = 0.0;
-        for (int r = jp + 1 + lane; r < n; r += 64) s += col[r] * v[r];
-        for (int o = 32; o > 0; o >>= 1) s += __shfl_down(s, o, 64);
-        if (lane == 0) xs[q] = s;
-      }
-      __syncthreads();
-      double part = 0.0;
-      for (int r = t; r < n; r += nt) {
-        double val = 0.0;
-        if (r > jp) {
-          double acc = y[r];
-          for (int q = 0; q < ifin; ++q)
-            acc -= Vp[r + (size_t)q * ld] * xs[ifin + q] + Wp[r + (size_t)q * ld] * xs[q];
-          val = tj * acc;
-          part += val * v[r];
-        }
-        w[r] = val;
-      }
-      double wdv = block_sum(part, red);
-      double a2 = -0.5 * tj * wdv;
-      for (int r = jp + 1 + t; r < n; r += nt) w[r] += a2 * v[r];
-    } else {
-      for (int r = t; r < n; r += nt) w[r] = 0.0;
-    }
-    __syncthreads();
+// Per column j (panel column i) two kernels run across the whole GPU:
+//  trd_reflect_symv: every block forms the reflector of column j from the
+//    norm partials left by the previous kernel (identically in every block);
+//    its waves compute y = A22 v (one wave per trailing column, from the
+//    unscaled column and the scale), the panel dots a = W^T v and
+//    b = V^T v, partial sums of y^T v, and store v into the panel.
+//  trd_post: w = tau (y - V a - W b) + alpha v, where
+//    alpha = -tau/2 w^T v = -tau^2/2 (y^T v - 2 a^T b) needs no extra
+//    reduction pass; v goes into A; then the next panel column's update
+//    A(:,j+1) -= V W(j+1,:)^T + W V(j+1,:)^T and the partial sums of its
+//    squared tail for the next reflector.
+constexpr int TRD_ROWS = 64;   // rows per block of the row-parallel kernels
+
+__device__ __forceinline__ double wave_sum(double v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// sum over a 256-thread block, the same value returned to every thread and
+// computed in the same order in every block (bitwise identical results)
+__device__ __forceinline__ double block_sum256(double v, double* sh4) {
+  v = wave_sum(v);
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) sh4[threadIdx.x >> 6] = v;
+  __syncthreads();
+  return (sh4[0] + sh4[1]) + (sh4[2] + sh4[3]);
+}
+
+__device__ __forceinline__ double sum_partials(const double* __restrict__ p, int np, double* sh4) {
+  double s = 0.0;
+  for (int b = threadIdx.x; b < np; b += 256) s += p[b];
+  return block_sum256(s, sh4);
+}
+
+// partial sums of A[r, j]^2, r >= j+2, one per TRD_ROWS-row block
+__global__ __launch_bounds__(256) void trd_colnorm_kernel(const double* __restrict__ A, int n, int j,
+                                                          double* __restrict__ pnorm) {
+  __shared__ double sh4[4];
+  int r = blockIdx.x * TRD_ROWS + threadIdx.x;
+  double s = 0.0;
+  if (threadIdx.x < TRD_ROWS && r < n && r >= j + 2) {
+    double a = A[r + (size_t)j * n];
+    s = a * a;
   }
-  // update column j = p + icol with the panel so far, then its reflector
-  if (icol >= 0) {
-    int j = p + icol;
-    double* a = A + (size_t)j * ld;
-    for (int q = t; q < icol; q += nt) {
-      xs[q] = Vp[j + (size_t)q * ld];
-      xs[NB + q] = Wp[j + (size_t)q * ld];
+  s = block_sum256(s, sh4);
+  if (threadIdx.x == 0) pnorm[blockIdx.x] = s;
+}
+
+__global__ __launch_bounds__(256) void trd_reflect_symv_kernel(
+    const double* __restrict__ A, int n, int j, int i, const double* __restrict__ pnorm, int npn,
+    double* __restrict__ Vp, const double* __restrict__ Wp, double* __restrict__ y,
+    double* __restrict__ ab, double* __restrict__ pyv, double* d, double* e, double* tau) {
+  __shared__ double sh4[4];
+  __shared__ double shc[4];
+  const size_t ld = (size_t)n;
+  const double* a = A + (size_t)j * ld;
+  double xn2 = sum_partials(pnorm, npn, sh4);
+  double alpha = a[j + 1];
+  double beta = alpha, tv = 0.0, scale = 1.0;
+  if (xn2 != 0.0) {
+    beta = -copysign(sqrt(alpha * alpha + xn2), alpha);
+    tv = (beta - alpha) / beta;
+    scale = 1.0 / (alpha - beta);
+  }
+  int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  int gw = blockIdx.x * 4 + wv;
+  int m = n - j - 1;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    d[j] = a[j];
+    e[j] = beta;
+    tau[j] = tv;
+  }
+  for (int r = blockIdx.x * 256 + threadIdx.x; r < n; r += gridDim.x * 256) {
+    double v = r <= j ? 0.0 : (r == j + 1 ? 1.0 : a[r] * scale);
+    Vp[r + (size_t)i * ld] = v;
+  }
+  double contrib = 0.0;
+  if (gw < m) {
+    int c = j + 1 + gw;
+    const double* col = A + (size_t)c * ld;
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+    int r = j + 2 + lane;
+    for (; r + 192 < n; r += 256) {
+      s0 += col[r] * a[r];
+      s1 += col[r + 64] * a[r + 64];
+      s2 += col[r + 128] * a[r + 128];
+      s3 += col[r + 192] * a[r + 192];
     }
-    __syncthreads();
-    for (int r = j + t; r < n; r += nt) {
-      double acc = a[r];
-      for (int q = 0; q < icol; ++q)
-        acc -= Vp[r + (size_t)q * ld] * xs[NB + q] + Wp[r + (size_t)q * ld] * xs[q];
-      a[r] = acc;
+    for (; r < n; r += 64) s0 += col[r] * a[r];
+    double s = wave_sum((s0 + s1) + (s2 + s3));
+    double yc = col[j + 1] + scale * s;
+    if (lane == 0) {
+      y[c] = yc;
+      contrib = yc * ((c == j + 1) ? 1.0 : a[c] * scale);
     }
-    __syncthreads();
-    if (t == 0) d[j] = a[j];
-    double* vcol = Vp + (size_t)icol * ld;
-    if (j < n - 1) {
-      double part = 0.0;
-      for (int r = j + 2 + t; r < n; r += nt) part += a[r] * a[r];
-      double xn2 = block_sum(part, red);
-      if (t == 0) {
-        double alpha = a[j + 1];
-        double beta = alpha, tv = 0.0, scale = 1.0;
-        if (xn2 != 0.0) {
-          beta = -copysign(sqrt(alpha * alpha + xn2), alpha);
-          tv = (beta - alpha) / beta;
-          scale = 1.0 / (alpha - beta);
-        }
-        sc[0] = scale;
-        e[j] = beta;
-        tau[j] = tv;
+  } else if (gw < m + 2 * i) {
+    int q = gw - m;
+    const double* col = (q < i) ? Wp + (size_t)q * ld : Vp + (size_t)(q - i) * ld;
+    double s = 0.0;
+    for (int r = j + 2 + lane; r < n; r += 64) s += col[r] * a[r];
+    s = wave_sum(s);
+    if (lane == 0) ab[q] = col[j + 1] + scale * s;
+  }
+  if (lane == 0) shc[wv] = contrib;
+  __syncthreads();
+  if (threadIdx.x == 0) pyv[blockIdx.x] = (shc[0] + shc[1]) + (shc[2] + shc[3]);
+}
+
+// rows [64 b, 64 b + 64): w = tau (y - V a - W b) + alpha v, v -> A(:, j),
+// next panel column update and its squared-tail partial sum
+__global__ __launch_bounds__(256) void trd_post_kernel(
+    double* __restrict__ A, int n, int j, int i, int nsym, const double* __restrict__ Vp,
+    double* __restrict__ Wp, const double* __restrict__ y, const double* __restrict__ ab,
+    const double* __restrict__ pyv, const double* __restrict__ tau, int next,
+    double* __restrict__ pnorm) {
+  __shared__ double Vs[NB][TRD_ROWS + 1];
+  __shared__ double Ws[NB][TRD_ROWS + 1];
+  __shared__ double abs_[2 * NB];
+  __shared__ double xs[2 * NB];
+  __shared__ double sh4[4];
+  __shared__ double wj1s;
+  const size_t ld = (size_t)n;
+  const int t = threadIdx.x;
+  const int r0 = blockIdx.x * TRD_ROWS;
+  const double tj = tau[j];
+  const double ytv = sum_partials(pyv, nsym, sh4);
+  if (t < 2 * i) abs_[t] = ab[t];
+  // stage V(:, 0..i) and W(:, 0..i-1) for this block's rows
+  for (int x = t; x < TRD_ROWS * (i + 1); x += 256) {
+    int rr = x % TRD_ROWS, q = x / TRD_ROWS, r = r0 + rr;
+    Vs[q][rr] = r < n ? Vp[r + (size_t)q * ld] : 0.0;
+    Ws[q][rr] = (r < n && q < i) ? Wp[r + (size_t)q * ld] : 0.0;
+  }
+  const int jn = j + 1;
+  if (next && t < i) {
+    xs[t] = Vp[jn + (size_t)t * ld];
+    xs[NB + t] = Wp[jn + (size_t)t * ld];
+  }
+  __syncthreads();
+  double atb = 0.0;
+  for (int q = 0; q < i; ++q) atb += abs_[q] * abs_[i + q];
+  const double alpha = -0.5 * tj * tj * (ytv - 2.0 * atb);
+  if (next && t == 0) {
+    // w(jn), needed by every block for the next column update
+    double wv = 0.0;
+    if (tj != 0.0) {
+      double acc = y[jn];
+      for (int q = 0; q < i; ++q) acc -= xs[q] * abs_[q] + xs[NB + q] * abs_[i + q];
+      wv = tj * acc + alpha;    // v(jn) = 1
+    }
+    wj1s = wv;
+  }
+  __syncthreads();
+  double s = 0.0;
+  if (t < TRD_ROWS) {
+    int r = r0 + t;
+    if (r < n) {
+      double vr = Vs[i][t];
+      double wr = 0.0;
+      if (r > j && tj != 0.0) {
+        double acc = y[r];
+        for (int q = 0; q < i; ++q) acc -= Vs[q][t] * abs_[q] + Ws[q][t] * abs_[i + q];
+        wr = tj * acc + alpha * vr;
       }
-      __syncthreads();
-      double scale = sc[0];
-      for (int r = t; r < n; r += nt) {
-        double v;
-        if (r <= j) v = 0.0;
-        else if (r == j + 1) v = 1.0;
-        else { v = a[r] * scale; a[r] = v; }
-        vcol[r] = v;
+      Wp[r + (size_t)i * ld] = wr;
+      if (r > j) A[r + (size_t)j * ld] = vr;
+      if (next && r >= jn) {
+        double acc = A[r + (size_t)jn * ld];
+        for (int q = 0; q < i; ++q) acc -= Vs[q][t] * xs[NB + q] + Ws[q][t] * xs[q];
+        acc -= vr * wj1s + wr;   // panel column i: V(jn, i) = 1
+        A[r + (size_t)jn * ld] = acc;
+        if (r >= jn + 2) s = acc * acc;
       }
-      __syncthreads();
-      if (t == 0) a[j + 1] = 1.0;
-    } else {
-      if (t == 0) tau[j] = 0.0;
-      for (int r = t; r < n; r += nt) vcol[r] = 0.0;
     }
+  }
+  if (next) {
+    s = block_sum256(s, sh4);
+    if (t == 0) pnorm[blockIdx.x] = s;
   }
 }
 
-// y[c] = sum_{r > j} A[r, c] v[r] for c > j (A symmetric in full storage)
-__global__ __launch_bounds__(256) void trd_symv_kernel(const double* __restrict__ A, int n, int j,
-                                                       const double* __restrict__ v,
-                                                       double* __restrict__ y) {
-  int lane = threadIdx.x & 63;
-  int c = j + 1 + blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (c >= n) return;
-  const double* col = A + (size_t)c * n;
-  double s0 = 0.0, s1 = 0.0;
-  int r = j + 1 + lane;
-  for (; r + 64 < n; r += 128) {
-    s0 += col[r] * v[r];
-    s1 += col[r + 64] * v[r + 64];
-  }
-  if (r < n) s0 += col[r] * v[r];
-  double s = s0 + s1;
-  for (int o = 32; o > 0; o >>= 1) s += __shfl_down(s, o, 64);
-  if (lane == 0) y[c] = s;
+__global__ void trd_last_kernel(const double* __restrict__ A, int n, double* d, double* tau) {
+  d[n - 1] = A[(size_t)(n - 1) * n + (n - 1)];
+  tau[n - 1] = 0.0;
 }
 
 // --------------------------------------------------------- back-transform
@@ -490,22 +556,31 @@ static int eig_workspace(itg_ctx* ctx, size_t bytes, void** out) {
 // Tridiagonalise A (n x n FP64, full symmetric storage) in place: d, e out;
 // reflectors left in A's strictly lower part (v_j(j+1) = 1 explicit), tau out.
 static int tridiagonalize(itg_ctx* ctx, int n, double* A, double* d, double* e, double* tau,
-                          double* Vp, double* Wp, double* y) {
+                          double* Vp, double* Wp, double* y, double* ab, double* pnorm,
+                          double* pyv) {
   for (int p = 0; p < n; p += NB) {
     int nbp = std::min(NB, n - p);
+    const int npn = (n + TRD_ROWS - 1) / TRD_ROWS;
+    if (p < n - 1) {
+      trd_colnorm_kernel<<<npn, 256, 0, ctx->stream>>>(A, n, p, pnorm);
+      ITG_LAUNCH_CHECK(ctx);
+    }
     for (int i = 0; i < nbp; ++i) {
       int j = p + i;
-      trd_step_kernel<<<1, 1024, 0, ctx->stream>>>(A, n, p, i > 0 ? i - 1 : -1, i, Vp, Wp, y, d,
-                                                   e, tau);
-      ITG_LAUNCH_CHECK(ctx);
-      if (j < n - 1) {
-        int m = n - j - 1;
-        trd_symv_kernel<<<(m + 3) / 4, 256, 0, ctx->stream>>>(A, n, j, Vp + (size_t)i * n, y);
+      if (j == n - 1) {
+        trd_last_kernel<<<1, 1, 0, ctx->stream>>>(A, n, d, tau);
         ITG_LAUNCH_CHECK(ctx);
+        break;
       }
+      int m = n - j - 1;
+      int nsym = (m + 2 * i + 3) / 4;
+      trd_reflect_symv_kernel<<<nsym, 256, 0, ctx->stream>>>(A, n, j, i, pnorm, npn, Vp, Wp, y,
+                                                             ab, pyv, d, e, tau);
+      int next = (i + 1 < nbp) ? 1 : 0;
+      trd_post_kernel<<<npn, 256, 0, ctx->stream>>>(A, n, j, i, nsym, Vp, Wp, y, ab, pyv, tau,
+                                                    next, pnorm);
+      ITG_LAUNCH_CHECK(ctx);
     }
-    trd_step_kernel<<<1, 1024, 0, ctx->stream>>>(A, n, p, nbp - 1, -1, Vp, Wp, y, d, e, tau);
-    ITG_LAUNCH_CHECK(ctx);
     int q = p + nbp;
     if (q < n) {
       int m2 = n - q;
@@ -635,7 +710,7 @@ extern "C" int itg_eigh_trunc(itg_ctx* ctx, int dtype, int64_t n64, void* Ain, d
   size_t nn = (size_t)n * n;
   bool f32 = dtype == ITG_F32;
   size_t bytes = (3 * nn + (f32 ? 2 * nn : 0)) * sizeof(double) + 2 * (size_t)n * NB * 8 +
-                 (size_t)n * 8 * 24 + 4 * (size_t)n * NB * 8 + 4 * NB * NB * 8 + 64 * 1024 +
+                 (size_t)n * 8 * 26 + 4 * (size_t)n * NB * 8 + 4 * NB * NB * 8 + 64 * 1024 +
                  64 * 256;
   void* ws;
   rc = eig_workspace(ctx, bytes, &ws);
@@ -649,6 +724,9 @@ extern "C" int itg_eigh_trunc(itg_ctx* ctx, int dtype, int64_t n64, void* Ain, d
   double* Vp = cv.take<double>((size_t)n * NB);
   double* Wp = cv.take<double>((size_t)n * NB);
   double* y = cv.take<double>(n);
+  double* ab = cv.take<double>(2 * NB);
+  double* pnorm = cv.take<double>((size_t)n / TRD_ROWS + 1);
+  double* pyv = cv.take<double>((size_t)n / 4 + 2 * NB);
   double* d = cv.take<double>(n);
   double* e = cv.take<double>(n);
   double* tau = cv.take<double>(n);
@@ -673,7 +751,7 @@ extern "C" int itg_eigh_trunc(itg_ctx* ctx, int dtype, int64_t n64, void* Ain, d
     if (rc) return rc;
   }
   // 1. tridiagonalise
-  rc = tridiagonalize(ctx, n, A, d, e, tau, Vp, Wp, y);
+  rc = tridiagonalize(ctx, n, A, d, e, tau, Vp, Wp, y, ab, pnorm, pyv);
   if (rc) return rc;
   // 2. divide and conquer
   double *Q, *D;

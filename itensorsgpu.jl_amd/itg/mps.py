@@ -14,9 +14,11 @@ examples/dmrg.jl:25), with every FLOP in libitg:
 Tensor index orders: MPS site (l, s, r) with dimension-1 boundary links;
 MPO site (wl, wr, s', s) with wl = 1 on the first and wr = 1 on the last site;
 environments L (a', w, a) and R (b', w, b).
-`mode` selects the arithmetic of the Heff and environment contractions
-("native" = FP64 MFMA, "fp32", "bf16x3"); storage, Davidson vector algebra,
-the density matrix, its eigendecomposition and the energy stay FP64.
+`mode` selects the arithmetic of the Heff applies ("native" = FP64 MFMA,
+"fp32", "bf16x3"); storage, environments (whose errors would accumulate
+along the chain), Davidson vector algebra, the density matrix, its
+eigendecomposition and the reported energy (an FP64 Rayleigh quotient at the
+end of each sweep) stay FP64.
 """
 import math
 import time
@@ -199,7 +201,9 @@ def random_mps(N, d, chi, seed, dtype="float64"):
     data, shp, off = [], [], 0
     for j in range(N):
         n = dims[j] * d * dims[j + 1]
-        data.append(ops.randn(n, seed, off, dtype))
+        t = ops.randn(n, seed, off, dtype)
+        ops.scal(t, 1.0 / ops.nrm2(t))          # normalize!(M[i]), cumps.jl:48
+        data.append(t)
         shp.append((dims[j], d, dims[j + 1]))
         off += n
     psi = MPS(data, shp)
@@ -222,6 +226,9 @@ def right_orthogonalize(psi):
         pl, pd, _ = psi.dims[j - 1]
         out = ops.empty(pl * pd * k, str(A.dtype).split(".")[-1])
         ops.gemm("N", "T", pl * pd, k, dl, 1.0, psi.data[j - 1], pl * pd, R, k, 0.0, out, pl * pd)
+        # keep the running product O(1): QR is invariant under positive
+        # scaling, and the centre is normalised at the end
+        ops.scal(out, 1.0 / ops.nrm2(out))
         psi.data[j - 1] = out
         psi.dims[j - 1] = (pl, pd, k)
     psi.llim, psi.rlim = 0, 2
@@ -291,9 +298,10 @@ def update_right(R, Rd, A, Ad, W, Wd, mode="native"):
 class ProjMPO:
     """Left/right environments of H around a two-site window."""
 
-    def __init__(self, H, mode="native", timer=None):
+    def __init__(self, H, mode="native", timer=None, env_mode="native"):
         self.H = H
-        self.mode = mode
+        self.mode = mode            # Heff apply arithmetic
+        self.env_mode = env_mode    # environment update arithmetic
         self.timer = timer
         N = len(H)
         self.L = [None] * (N + 1)
@@ -305,7 +313,7 @@ class ProjMPO:
         for j in range(N - 1, upto - 1, -1):
             R, Rd = self.R[j + 1]
             self.R[j] = update_right(R, Rd, psi.data[j], psi.dims[j], self.H.data[j],
-                                     self.H.dims[j], self.mode)
+                                     self.H.dims[j], self.env_mode)
         self.L[0] = ones_env()
 
     def product(self, b, x, xdims, out=None, work=None):
@@ -486,13 +494,15 @@ def dmrg(H, psi, sweeps, mode="native", krylovdim=3, precond=None, errgoal=1e-14
     previous call continues from its environments (a completed sweep leaves
     psi right-orthogonal and the right environments current)."""
     N = len(psi)
-    env_mode = mode if env_mode is None else env_mode
+    # environments accumulate along the chain (L_b is b successive updates),
+    # so their errors add up: they stay FP64 unless asked otherwise
+    env_mode = "native" if env_mode is None else env_mode
     if ph is None:
-        PH = ProjMPO(H, env_mode, timer)
+        PH = ProjMPO(H, mode, timer, env_mode)
         PH.build_right(psi)
     else:
         PH = ph
-        PH.mode, PH.timer = env_mode, timer
+        PH.mode, PH.env_mode, PH.timer = mode, env_mode, timer
     dmrg.last_projmpo = PH
     energy = 0.0
     for sw in range(sweeps.nsweep):
@@ -537,8 +547,20 @@ def dmrg(H, psi, sweeps, mode="native", krylovdim=3, precond=None, errgoal=1e-14
                                            H.data[b + 1], H.dims[b + 1], env_mode)
             if ev is not None:
                 timer.stop("environment", ev)
+        lam = energy
+        if mode != "native":
+            # FP64 Rayleigh quotient of the current MPS at the first bond:
+            # L[0] is the boundary and R[2] was rebuilt in FP64 from the
+            # current tensors, so this is <psi|H|psi>/<psi|psi> exactly
+            phi, pdims = ops.contract(psi.data[0], psi.dims[0], [1, 2, 3], psi.data[1],
+                                      psi.dims[1], [3, 4, 5], [1, 2, 4, 5])
+            saved = PH.mode
+            PH.mode = "native"
+            y = PH.product(0, phi, pdims)
+            PH.mode = saved
+            energy = ops.dot(phi, y) / ops.dot(phi, phi)
         _lib.ctx().sync()
-        rec = {"sweep": sw + 1, "energy": energy, "maxerr": maxerr,
+        rec = {"sweep": sw + 1, "energy": energy, "energy_davidson": lam, "maxerr": maxerr,
                "maxlinkdim": psi.maxlinkdim(), "seconds": time.perf_counter() - t0,
                "matvecs": stats.get("matvecs", 0)}
         if record is not None:

@@ -31,6 +31,7 @@ def random_mps(N, d, chi, seed):
     for j in range(N):
         n = dims[j] * d * dims[j + 1]
         x = philox.randn(n, seed, off)
+        x = x / np.linalg.norm(x)           # normalize!(M[i]), cumps.jl:48
         off += n
         mps.append(np.reshape(x, (dims[j], d, dims[j + 1]), order="F"))
     right_orthogonalize(mps)
@@ -48,6 +49,7 @@ def right_orthogonalize(mps):
         k = Q.shape[1]
         mps[j] = np.reshape(Q.T, (k, d, dr), order="F")
         mps[j - 1] = np.einsum("lsr,rk->lsk", mps[j - 1], R.T)
+        mps[j - 1] = mps[j - 1] / np.linalg.norm(mps[j - 1])
 
 
 def left_orthonormal_error(A):
