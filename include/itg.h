@@ -222,6 +222,22 @@ int itg_mlp_backward(itg_ctx* ctx, int rows, int in, int hidden, int out,
 int itg_mlp_adam(itg_ctx* ctx, int64_t nparam, float* params, const float* grad,
                  float* m, float* v, int step, float lr, float beta1,
                  float beta2, float eps);
+/* Feature rows of a two-site FP64 residual r (chiL,d,d,chiR): one row per
+ * bond pair (a,b), row index a + chiL*b; rows row0 + k*row_step for
+ * k < nrows. Each row holds 3*d*d floats: r/rms(r), delta/rms(delta) and
+ * (r/delta)/rms(r/delta) over the d*d physical block, delta = diag - lambda
+ * clipped to |delta| >= floor. The rms scales are taken over all of r. */
+int itg_mlp_features(itg_ctx* ctx, int64_t chiL, int64_t d, int64_t chiR,
+                     const double* r, const double* diag, double lambda,
+                     double floor, int64_t row0, int64_t row_step,
+                     int64_t nrows, float* X);
+/* The same rows of an FP64 vector v, times scale, as d*d floats per row. */
+int itg_mlp_gather(itg_ctx* ctx, int64_t chiL, int64_t d, int64_t chiR,
+                   const double* v, double scale, int64_t row0,
+                   int64_t row_step, int64_t nrows, float* T);
+/* Inverse of itg_mlp_gather for the contiguous rows row0..row0+nrows-1. */
+int itg_mlp_scatter(itg_ctx* ctx, int64_t chiL, int64_t d, int64_t chiR,
+                    const float* Y, int64_t row0, int64_t nrows, double* v);
 
 #ifdef __cplusplus
 }

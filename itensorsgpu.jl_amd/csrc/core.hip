@@ -65,6 +65,7 @@ extern "C" int itg_destroy(itg_ctx* ctx) {
   itg_plan_cache_clear(ctx);
   if (ctx->ws) hipFree(ctx->ws);
   if (ctx->eig_ws) hipFree(ctx->eig_ws);
+  if (ctx->mlp_ws) hipFree(ctx->mlp_ws);
   if (ctx->red) hipFree(ctx->red);
   if (ctx->red_host) hipHostFree(ctx->red_host);
   hipStreamDestroy(ctx->own_stream);

@@ -27,6 +27,9 @@ struct itg_ctx {
   // grow-only device workspace of the factorisations (eigh/svd/qr)
   void* eig_ws = nullptr;
   size_t eig_ws_bytes = 0;
+  // grow-only device workspace of the MLP backward pass
+  void* mlp_ws = nullptr;
+  size_t mlp_ws_bytes = 0;
   // small device scratch for reductions + pinned host mirror
   double* red = nullptr;        // device, kRedSlots doubles
   double* red_host = nullptr;   // pinned host, kRedSlots doubles

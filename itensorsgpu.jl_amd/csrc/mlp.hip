@@ -1,0 +1,410 @@
+// 3-layer GELU MLP (in -> h -> h -> out) for the learned Davidson
+// preconditioner: forward, MSE backward and Adam, FP32. No counterpart in the
+// reference. Activations are feature-major (column-major features x rows, i.e.
+// row-major rows x features); weights are packed row-major
+// [W1(h x in), b1, W2(h x h), b2, W3(out x h), b3]. The matrix products run
+// on FP32 MFMA through the contraction kernel; bias, GELU and the loss are
+// fused elementwise kernels.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+
+#include "itg_internal.h"
+
+namespace {
+
+__device__ __forceinline__ float gelu_f(float x) {
+  return 0.5f * x * (1.0f + erff(x * 0.70710678118654752f));
+}
+__device__ __forceinline__ float gelu_grad_f(float x) {
+  return 0.5f * (1.0f + erff(x * 0.70710678118654752f)) +
+         x * 0.39894228040143268f * expf(-0.5f * x * x);
+}
+
+// Z (f x rows) += b[f]; A = gelu(Z) (A may be null: bias only)
+__global__ void bias_act_kernel(int f, int64_t rows, float* __restrict__ Z, const float* __restrict__ b,
+                                float* __restrict__ A) {
+  int64_t n = (int64_t)f * rows;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    float z = Z[e] + b[e % f];
+    Z[e] = z;
+    if (A) A[e] = gelu_f(z);
+  }
+}
+
+// dY = 2 (Y - T) / (rows * out); per-block partial sums of (Y - T)^2
+__global__ __launch_bounds__(256) void mse_grad_kernel(int64_t n, const float* __restrict__ Y,
+                                                       const float* __restrict__ T, float scale,
+                                                       float* __restrict__ dY, double* part) {
+  __shared__ double sh[4];
+  double s = 0.0;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    float dlt = Y[e] - T[e];
+    dY[e] = 2.0f * dlt * scale;
+    s += (double)dlt * dlt;
+  }
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_down(s, o, 64);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) part[blockIdx.x] = sh[0] + sh[1] + sh[2] + sh[3];
+}
+
+__global__ void sum_partials_kernel(int n, const double* part, double* out) {
+  __shared__ double sh[4];
+  double s = 0.0;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) s += part[i];
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_down(s, o, 64);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) out[0] = sh[0] + sh[1] + sh[2] + sh[3];
+}
+
+// dZ = dA * gelu'(Z), in place on dA
+__global__ void gelu_back_kernel(int64_t n, const float* __restrict__ Z, float* __restrict__ dA) {
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n;
+       e += (int64_t)gridDim.x * blockDim.x)
+    dA[e] *= gelu_grad_f(Z[e]);
+}
+
+// gb[f] = sum over rows of dZ(f, row): one block per feature
+__global__ __launch_bounds__(256) void bias_grad_kernel(int f, int64_t rows, const float* __restrict__ dZ,
+                                                        float* __restrict__ gb) {
+  __shared__ double sh[4];
+  int feat = blockIdx.x;
+  double s = 0.0;
+  for (int64_t r = threadIdx.x; r < rows; r += blockDim.x) s += dZ[feat + (int64_t)f * r];
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_down(s, o, 64);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) gb[feat] = (float)(sh[0] + sh[1] + sh[2] + sh[3]);
+}
+
+__global__ void adam_kernel(int64_t n, float* __restrict__ p, const float* __restrict__ g,
+                            float* __restrict__ m, float* __restrict__ v, float lr, float b1,
+                            float b2, float eps, float c1, float c2) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    float gi = g[i];
+    float mi = b1 * m[i] + (1.0f - b1) * gi;
+    float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    p[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+  }
+}
+
+struct Layout {
+  int64_t W1, b1, W2, b2, W3, b3, total;
+};
+
+Layout layout(int in, int h, int out) {
+  Layout L;
+  L.W1 = 0;
+  L.b1 = L.W1 + (int64_t)h * in;
+  L.W2 = L.b1 + h;
+  L.b2 = L.W2 + (int64_t)h * h;
+  L.W3 = L.b2 + h;
+  L.b3 = L.W3 + (int64_t)out * h;
+  L.total = L.b3 + out;
+  return L;
+}
+
+int check_shape(itg_ctx* ctx, int rows, int in, int h, int out) {
+  if (rows <= 0 || in <= 0 || h <= 0 || out <= 0)
+    return itg_fail(ctx, ITG_EDIM, "mlp: non-positive dimension");
+  return ITG_OK;
+}
+
+}  // namespace
+
+extern "C" int64_t itg_mlp_param_count(int in, int hidden, int out) {
+  return layout(in, hidden, out).total;
+}
+
+extern "C" int itg_mlp_forward(itg_ctx* ctx, int rows, int in, int h, int out, const float* P,
+                               const float* X, float* Y, float* acts) {
+  ITG_CHECK_CTX(ctx);
+  int rc = check_shape(ctx, rows, in, h, out);
+  if (rc) return rc;
+  Layout L = layout(in, h, out);
+  int64_t rh = (int64_t)rows * h;
+  float* Z1 = acts;
+  float* A1 = acts + rh;
+  float* Z2 = acts + 2 * rh;
+  float* A2 = acts + 3 * rh;
+  rc = itg_gemm_internal(ctx, ITG_F32, ITG_MODE_NATIVE, 'T', 'N', h, rows, in, 1.0, P + L.W1, in,
+                         X, in, 0.0, Z1, h);
+  if (rc) return rc;
+  bias_act_kernel<<<itg_blocks(rh, 256, 8192), 256, 0, ctx->stream>>>(h, rows, Z1, P + L.b1, A1);
+  rc = itg_gemm_internal(ctx, ITG_F32, ITG_MODE_NATIVE, 'T', 'N', h, rows, h, 1.0, P + L.W2, h, A1,
+                         h, 0.0, Z2, h);
+  if (rc) return rc;
+  bias_act_kernel<<<itg_blocks(rh, 256, 8192), 256, 0, ctx->stream>>>(h, rows, Z2, P + L.b2, A2);
+  rc = itg_gemm_internal(ctx, ITG_F32, ITG_MODE_NATIVE, 'T', 'N', out, rows, h, 1.0, P + L.W3, h,
+                         A2, h, 0.0, Y, out);
+  if (rc) return rc;
+  bias_act_kernel<<<itg_blocks((int64_t)rows * out, 256, 8192), 256, 0, ctx->stream>>>(
+      out, rows, Y, P + L.b3, nullptr);
+  ITG_LAUNCH_CHECK(ctx);
+  return ITG_OK;
+}
+
+static int mlp_workspace(itg_ctx* ctx, size_t bytes, void** out) {
+  if (bytes > ctx->mlp_ws_bytes) {
+    if (ctx->mlp_ws) {
+      ITG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+      ITG_HIP(ctx, hipFree(ctx->mlp_ws));
+      ctx->mlp_ws = nullptr;
+      ctx->mlp_ws_bytes = 0;
+    }
+    if (hipMalloc(&ctx->mlp_ws, bytes) != hipSuccess) {
+      hipGetLastError();
+      return itg_fail(ctx, ITG_EOOM, "mlp workspace of %zu bytes failed", bytes);
+    }
+    ctx->mlp_ws_bytes = bytes;
+  }
+  *out = ctx->mlp_ws;
+  return ITG_OK;
+}
+
+extern "C" int itg_mlp_backward(itg_ctx* ctx, int rows, int in, int h, int out, const float* P,
+                                const float* X, const float* T, const float* acts, const float* Y,
+                                float* G, double* loss) {
+  ITG_CHECK_CTX(ctx);
+  int rc = check_shape(ctx, rows, in, h, out);
+  if (rc) return rc;
+  Layout L = layout(in, h, out);
+  int64_t rh = (int64_t)rows * h, ro = (int64_t)rows * out;
+  const float* Z1 = acts;
+  const float* A1 = acts + rh;
+  const float* Z2 = acts + 2 * rh;
+  const float* A2 = acts + 3 * rh;
+  void* ws;
+  size_t nb = 1024;
+  rc = mlp_workspace(ctx, (size_t)(ro + 2 * rh) * sizeof(float) + nb * sizeof(double) + 256, &ws);
+  if (rc) return rc;
+  float* dY = (float*)ws;
+  float* dA2 = dY + ro;
+  float* dA1 = dA2 + rh;
+  double* part = (double*)(((uintptr_t)(dA1 + rh) + 255) & ~(uintptr_t)255);
+  unsigned nblk = itg_blocks(ro, 256, nb);
+  mse_grad_kernel<<<nblk, 256, 0, ctx->stream>>>(ro, Y, T, 1.0f / (float)ro, dY, part);
+  sum_partials_kernel<<<1, 256, 0, ctx->stream>>>((int)nblk, part, ctx->red);
+  ITG_LAUNCH_CHECK(ctx);
+  // layer 3
+  rc = itg_gemm_internal(ctx, ITG_F32, ITG_MODE_NATIVE, 'N', 'T', h, out, rows, 1.0, A2, h, dY, out,
+                         0.0, G + L.W3, h);
+  if (rc) return rc;
+  bias_grad_kernel<<<out, 256, 0, ctx->stream>>>(out, rows, dY, G + L.b3);
+  rc = itg_gemm_internal(ctx, ITG_F32, ITG_MODE_NATIVE, 'N', 'N', h, rows, out, 1.0, P + L.W3, h,
+                         dY, out, 0.0, dA2, h);
+  if (rc) return rc;
+  gelu_back_kernel<<<itg_blocks(rh, 256, 8192), 256, 0, ctx->stream>>>(rh, Z2, dA2);
+  // layer 2
+  rc = itg_gemm_internal(ctx, ITG_F32, ITG_MODE_NATIVE, 'N', 'T', h, h, rows, 1.0, A1, h, dA2, h,
+                         0.0, G + L.W2, h);
+  if (rc) return rc;
+  bias_grad_kernel<<<h, 256, 0, ctx->stream>>>(h, rows, dA2, G + L.b2);
+  rc = itg_gemm_internal(ctx, ITG_F32, ITG_MODE_NATIVE, 'N', 'N', h, rows, h, 1.0, P + L.W2, h,
+                         dA2, h, 0.0, dA1, h);
+  if (rc) return rc;
+  gelu_back_kernel<<<itg_blocks(rh, 256, 8192), 256, 0, ctx->stream>>>(rh, Z1, dA1);
+  // layer 1
+  rc = itg_gemm_internal(ctx, ITG_F32, ITG_MODE_NATIVE, 'N', 'T', in, h, rows, 1.0, X, in, dA1, h,
+                         0.0, G + L.W1, in);
+  if (rc) return rc;
+  bias_grad_kernel<<<h, 256, 0, ctx->stream>>>(h, rows, dA1, G + L.b1);
+  ITG_LAUNCH_CHECK(ctx);
+  if (loss) {
+    ITG_HIP(ctx, hipMemcpyAsync(ctx->red_host, ctx->red, sizeof(double), hipMemcpyDeviceToHost,
+                                ctx->stream));
+    ITG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    *loss = ctx->red_host[0] / (double)ro;
+  }
+  return ITG_OK;
+}
+
+extern "C" int itg_mlp_adam(itg_ctx* ctx, int64_t n, float* p, const float* g, float* m, float* v,
+                            int step, float lr, float b1, float b2, float eps) {
+  ITG_CHECK_CTX(ctx);
+  if (n <= 0) return ITG_OK;
+  if (step < 1) return itg_fail(ctx, ITG_EARG, "adam: step must be >= 1");
+  float c1 = 1.0f - powf(b1, (float)step), c2 = 1.0f - powf(b2, (float)step);
+  adam_kernel<<<itg_blocks(n, 256, 4096), 256, 0, ctx->stream>>>(n, p, g, m, v, lr, b1, b2, eps, c1,
+                                                                 c2);
+  ITG_LAUNCH_CHECK(ctx);
+  return ITG_OK;
+}
+
+// ------------------------------------------------------ residual features
+// The network sees a two-site vector (chiL, d, d, chiR) as chiL*chiR rows, one
+// per bond pair (a, b), each holding the d*d physical block. Row k has 3*d*d
+// features: r/rms(r), delta/rms(delta) and (r/delta)/rms(r/delta), with
+// delta = diag(Heff) - lambda clipped away from zero like the diagonal
+// preconditioner. Scales are global, so the map is the same at every chi.
+namespace {
+
+__device__ __forceinline__ double clip_den(double den, double fl) {
+  if (fabs(den) < fl) den = den < 0.0 ? -fl : fl;
+  return den;
+}
+
+__global__ __launch_bounds__(256) void feat_stats_kernel(int64_t n, const double* __restrict__ r,
+                                                         const double* __restrict__ dg, double lam,
+                                                         double fl, double* part) {
+  __shared__ double sh[3][4];
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    double re = r[e], de = dg[e] - lam, dc = clip_den(de, fl);
+    double p = re / dc;
+    s0 += re * re;
+    s1 += de * de;
+    s2 += p * p;
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    s0 += __shfl_down(s0, o, 64);
+    s1 += __shfl_down(s1, o, 64);
+    s2 += __shfl_down(s2, o, 64);
+  }
+  int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) {
+    sh[0][w] = s0;
+    sh[1][w] = s1;
+    sh[2][w] = s2;
+  }
+  __syncthreads();
+  if (threadIdx.x < 3) {
+    int c = threadIdx.x;
+    part[3 * blockIdx.x + c] = sh[c][0] + sh[c][1] + sh[c][2] + sh[c][3];
+  }
+}
+
+// inverse rms of each statistic (1 where it vanishes)
+__global__ void feat_scale_kernel(int nb, int64_t n, const double* part, double* inv) {
+  __shared__ double sh[3][4];
+  double s[3] = {0.0, 0.0, 0.0};
+  for (int i = threadIdx.x; i < nb; i += blockDim.x)
+    for (int c = 0; c < 3; ++c) s[c] += part[3 * i + c];
+  int w = threadIdx.x >> 6;
+  for (int c = 0; c < 3; ++c) {
+    for (int o = 32; o > 0; o >>= 1) s[c] += __shfl_down(s[c], o, 64);
+    if ((threadIdx.x & 63) == 0) sh[c][w] = s[c];
+  }
+  __syncthreads();
+  if (threadIdx.x < 3) {
+    int c = threadIdx.x;
+    double t = sh[c][0] + sh[c][1] + sh[c][2] + sh[c][3];
+    double rms = sqrt(t / (double)n);
+    inv[c] = rms > 0.0 ? 1.0 / rms : 1.0;
+  }
+}
+
+__global__ void features_kernel(int64_t chiL, int dd, int64_t chiR, const double* __restrict__ r,
+                                const double* __restrict__ dg, double lam, double fl,
+                                const double* __restrict__ inv, int64_t row0, int64_t step,
+                                int64_t nrows, float* __restrict__ X) {
+  double i0 = inv[0], i1 = inv[1], i2 = inv[2];
+  int64_t total = nrows * dd;
+  int nf = 3 * dd;
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    int64_t k = t / dd;
+    int j = (int)(t - k * dd);
+    int64_t row = row0 + k * step;
+    int64_t a = row % chiL, b = row / chiL;
+    int64_t e = a + chiL * (j + (int64_t)dd * b);
+    double re = r[e], de = dg[e] - lam;
+    float* x = X + k * nf;
+    x[j] = (float)(re * i0);
+    x[dd + j] = (float)(de * i1);
+    x[2 * dd + j] = (float)(re / clip_den(de, fl) * i2);
+  }
+}
+
+__global__ void gather_rows_kernel(int64_t chiL, int dd, int64_t chiR, const double* __restrict__ v,
+                                   double scale, int64_t row0, int64_t step, int64_t nrows,
+                                   float* __restrict__ T) {
+  int64_t total = nrows * dd;
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    int64_t k = t / dd;
+    int j = (int)(t - k * dd);
+    int64_t row = row0 + k * step;
+    int64_t a = row % chiL, b = row / chiL;
+    T[t] = (float)(v[a + chiL * (j + (int64_t)dd * b)] * scale);
+  }
+}
+
+__global__ void scatter_rows_kernel(int64_t chiL, int dd, int64_t chiR, const float* __restrict__ Y,
+                                    int64_t row0, int64_t nrows, double* __restrict__ v) {
+  int64_t total = nrows * dd;
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    int64_t k = t / dd;
+    int j = (int)(t - k * dd);
+    int64_t row = row0 + k;
+    int64_t a = row % chiL, b = row / chiL;
+    v[a + chiL * (j + (int64_t)dd * b)] = (double)Y[t];
+  }
+}
+
+int check_rows(itg_ctx* ctx, int64_t chiL, int64_t d, int64_t chiR, int64_t row0, int64_t step,
+               int64_t nrows) {
+  if (chiL <= 0 || d <= 0 || chiR <= 0 || d * d > 1024)
+    return itg_fail(ctx, ITG_EDIM, "mlp rows: bad shape (%lld,%lld,%lld)", (long long)chiL,
+                    (long long)d, (long long)chiR);
+  int64_t rows = chiL * chiR;
+  if (nrows < 0 || row0 < 0 || step < 1 || (nrows > 0 && row0 + (nrows - 1) * step >= rows))
+    return itg_fail(ctx, ITG_EDIM, "mlp rows: rows %lld + k*%lld (k < %lld) outside [0,%lld)",
+                    (long long)row0, (long long)step, (long long)nrows, (long long)rows);
+  return ITG_OK;
+}
+
+}  // namespace
+
+extern "C" int itg_mlp_features(itg_ctx* ctx, int64_t chiL, int64_t d, int64_t chiR,
+                                const double* r, const double* diag, double lambda, double floor,
+                                int64_t row0, int64_t row_step, int64_t nrows, float* X) {
+  ITG_CHECK_CTX(ctx);
+  int rc = check_rows(ctx, chiL, d, chiR, row0, row_step, nrows);
+  if (rc) return rc;
+  if (nrows == 0) return ITG_OK;
+  int64_t n = chiL * d * d * chiR;
+  unsigned nb = itg_blocks(n, 256, 1024);
+  double* part = ctx->red + 16;
+  double* inv = ctx->red + 8;
+  feat_stats_kernel<<<nb, 256, 0, ctx->stream>>>(n, r, diag, lambda, floor, part);
+  feat_scale_kernel<<<1, 256, 0, ctx->stream>>>((int)nb, n, part, inv);
+  features_kernel<<<itg_blocks(nrows * d * d, 256, 8192), 256, 0, ctx->stream>>>(
+      chiL, (int)(d * d), chiR, r, diag, lambda, floor, inv, row0, row_step, nrows, X);
+  ITG_LAUNCH_CHECK(ctx);
+  return ITG_OK;
+}
+
+extern "C" int itg_mlp_gather(itg_ctx* ctx, int64_t chiL, int64_t d, int64_t chiR, const double* v,
+                              double scale, int64_t row0, int64_t row_step, int64_t nrows,
+                              float* T) {
+  ITG_CHECK_CTX(ctx);
+  int rc = check_rows(ctx, chiL, d, chiR, row0, row_step, nrows);
+  if (rc) return rc;
+  if (nrows == 0) return ITG_OK;
+  gather_rows_kernel<<<itg_blocks(nrows * d * d, 256, 8192), 256, 0, ctx->stream>>>(
+      chiL, (int)(d * d), chiR, v, scale, row0, row_step, nrows, T);
+  ITG_LAUNCH_CHECK(ctx);
+  return ITG_OK;
+}
+
+extern "C" int itg_mlp_scatter(itg_ctx* ctx, int64_t chiL, int64_t d, int64_t chiR, const float* Y,
+                               int64_t row0, int64_t nrows, double* v) {
+  ITG_CHECK_CTX(ctx);
+  int rc = check_rows(ctx, chiL, d, chiR, row0, 1, nrows);
+  if (rc) return rc;
+  if (nrows == 0) return ITG_OK;
+  scatter_rows_kernel<<<itg_blocks(nrows * d * d, 256, 8192), 256, 0, ctx->stream>>>(
+      chiL, (int)(d * d), chiR, Y, row0, nrows, v);
+  ITG_LAUNCH_CHECK(ctx);
+  return ITG_OK;
+}

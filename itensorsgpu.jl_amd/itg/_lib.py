@@ -93,6 +93,11 @@ _SIGS = {
                                    _vp, _vp, _vp, _dp]),
     "itg_mlp_adam": (C.c_int, [_ctxp, C.c_int64, _vp, _vp, _vp, _vp, C.c_int, C.c_float,
                                C.c_float, C.c_float, C.c_float]),
+    "itg_mlp_features": (C.c_int, [_ctxp] + [C.c_int64] * 3 + [_vp, _vp, C.c_double, C.c_double]
+                         + [C.c_int64] * 3 + [_vp]),
+    "itg_mlp_gather": (C.c_int, [_ctxp] + [C.c_int64] * 3 + [_vp, C.c_double] + [C.c_int64] * 3
+                       + [_vp]),
+    "itg_mlp_scatter": (C.c_int, [_ctxp] + [C.c_int64] * 3 + [_vp, C.c_int64, C.c_int64, _vp]),
 }
 
 EXPORTED = tuple(_SIGS)

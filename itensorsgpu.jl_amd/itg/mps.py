@@ -340,11 +340,14 @@ class ProjMPO:
 
 # ----------------------------------------------------------------- davidson
 def davidson(matvec, x0, maxiter=2, miniter=1, errgoal=1e-14, northo=2, precond=None,
-             stats=None):
+             stats=None, fallback=None):
     """ITensors-style Davidson on device vectors. Returns (lambda, phi).
-    precond(q, lam) -> correction (device) or None; a preconditioned step
-    whose Ritz value does not decrease the residual norm is reverted to the
-    plain (unpreconditioned) direction."""
+    precond(q, lam) -> correction (device) or None for the plain residual.
+    With `fallback` (a preconditioner callable, or "plain"), a step taken
+    with `precond` that does not lower the residual norm is reverted (its
+    basis vector dropped, the previous Ritz pair restored) and taken again
+    with the fallback; stats["reverted"] counts such steps. The subspace,
+    its projected matrix and the Ritz values stay FP64 in every mode."""
     nrm = ops.nrm2(x0)
     phi = ops.copy(x0)
     ops.scal(phi, 1.0 / nrm)
@@ -356,50 +359,67 @@ def davidson(matvec, x0, maxiter=2, miniter=1, errgoal=1e-14, northo=2, precond=
     M = np.array([[lam]])
     last = float("nan")
     maxiter = min(maxiter, phi.numel() - 1)
-    nmv = 1
+    nmv, nrev = 1, 0
+    qn = ops.nrm2(q)
     for ni in range(1, maxiter + 1):
-        qn = ops.nrm2(q)
         conv = (qn < errgoal and abs(lam - last) < errgoal) or qn < max(1e-12, errgoal * 1e-3)
         if qn < 1e-20 or (conv and ni > miniter):
             break
         last = lam
-        t = None
-        if precond is not None:
-            t = precond(q, lam)
-        if t is None:
-            t = ops.copy(q)
-        for _ in range(northo):
-            for v in V:
-                ops.axpby(-ops.dot(v, t), v, 1.0, t)
-        tn = ops.nrm2(t)
-        if tn < 1e-14 * max(1.0, qn):
+        pres = [precond] if fallback is None else [precond, fallback]
+        for attempt, pre in enumerate(pres):
+            t = None
+            if pre is not None and pre != "plain":
+                t = pre(q, lam)
+            if t is None:
+                t = ops.copy(q)
+            for _ in range(northo):
+                for v in V:
+                    ops.axpby(-ops.dot(v, t), v, 1.0, t)
+            tn = ops.nrm2(t)
+            if tn < 1e-14 * max(1.0, qn):
+                if attempt + 1 < len(pres):
+                    nrev += 1
+                    continue
+                t = None
+                break
+            ops.scal(t, 1.0 / tn)
+            V.append(t)
+            AV.append(matvec(t))
+            nmv += 1
+            m = len(V)
+            Mn = np.zeros((m, m))
+            Mn[: m - 1, : m - 1] = M
+            for k in range(m):
+                Mn[k, m - 1] = Mn[m - 1, k] = ops.dot(V[k], AV[m - 1])
+            ev, U = np.linalg.eigh(Mn)
+            u = U[:, 0]
+            if u[0] < 0:
+                u = -u
+            nphi = ops.copy(V[0])
+            ops.scal(nphi, u[0])
+            nq = ops.copy(AV[0])
+            ops.scal(nq, u[0])
+            for k in range(1, m):
+                ops.axpby(u[k], V[k], 1.0, nphi)
+                ops.axpby(u[k], AV[k], 1.0, nq)
+            ops.axpby(-float(ev[0]), nphi, 1.0, nq)
+            nqn = ops.nrm2(nq)
+            if attempt + 1 < len(pres) and not nqn < qn:
+                # the guarded step did not lower the residual: drop it
+                V.pop()
+                AV.pop()
+                nrev += 1
+                continue
+            M, lam, phi, q, qn = Mn, float(ev[0]), nphi, nq, nqn
             break
-        ops.scal(t, 1.0 / tn)
-        V.append(t)
-        AV.append(matvec(t))
-        nmv += 1
-        m = len(V)
-        Mn = np.zeros((m, m))
-        Mn[: m - 1, : m - 1] = M
-        for k in range(m):
-            Mn[k, m - 1] = Mn[m - 1, k] = ops.dot(V[k], AV[m - 1])
-        M = Mn
-        ev, U = np.linalg.eigh(M)
-        lam = float(ev[0])
-        u = U[:, 0]
-        if u[0] < 0:
-            u = -u
-        phi = ops.copy(V[0])
-        ops.scal(phi, u[0])
-        q = ops.copy(AV[0])
-        ops.scal(q, u[0])
-        for k in range(1, m):
-            ops.axpby(u[k], V[k], 1.0, phi)
-            ops.axpby(u[k], AV[k], 1.0, q)
-        ops.axpby(-lam, phi, 1.0, q)
+        if t is None:
+            break
     ops.scal(phi, 1.0 / ops.nrm2(phi))
     if stats is not None:
         stats["matvecs"] = stats.get("matvecs", 0) + nmv
+        stats["reverted"] = stats.get("reverted", 0) + nrev
+        stats["residual"] = qn
     return lam, phi
 
 
@@ -528,7 +548,8 @@ def dmrg(H, psi, sweeps, mode="native", krylovdim=3, precond=None, errgoal=1e-14
             elif callable(precond):
                 pre = precond(PH, b, pdims)
             energy, phi = davidson(mv, phi, maxiter=krylovdim - 1, errgoal=errgoal,
-                                   precond=pre, stats=stats)
+                                   precond=pre, stats=stats,
+                                   fallback=getattr(pre, "fallback", None))
             ortho = "left" if ha == 1 else "right"
             ev = timer.start() if timer is not None else None
             terr, k = replacebond(psi, b, phi, pdims, ortho, sweeps.maxdim[sw],
@@ -562,7 +583,7 @@ def dmrg(H, psi, sweeps, mode="native", krylovdim=3, precond=None, errgoal=1e-14
         _lib.ctx().sync()
         rec = {"sweep": sw + 1, "energy": energy, "energy_davidson": lam, "maxerr": maxerr,
                "maxlinkdim": psi.maxlinkdim(), "seconds": time.perf_counter() - t0,
-               "matvecs": stats.get("matvecs", 0)}
+               "matvecs": stats.get("matvecs", 0), "reverted": stats.get("reverted", 0)}
         if record is not None:
             record.append(rec)
         if sweep_callback is not None:

@@ -272,3 +272,67 @@ def copy(t):
         c = ctx()
         c.check(lib().itg_memcpy_d2d(c.ptr, ptr(out), ptr(t), t.numel() * t.element_size()))
     return out
+
+
+# ------------------------------------------------- learned preconditioner
+def mlp_param_count(i, h, o):
+    return int(lib().itg_mlp_param_count(int(i), int(h), int(o)))
+
+
+def mlp_forward(P, X, rows, i, h, o, Y=None, acts=None):
+    """Y = MLP(X) for `rows` rows of `i` features; returns (Y, acts)."""
+    if Y is None:
+        Y = empty(rows * o, "float32")
+    if acts is None:
+        acts = empty(4 * rows * h, "float32")
+    c = ctx()
+    c.check(lib().itg_mlp_forward(c.ptr, int(rows), int(i), int(h), int(o), ptr(P), ptr(X), ptr(Y),
+                                  ptr(acts)))
+    return Y, acts
+
+
+def mlp_backward(P, X, T, acts, Y, rows, i, h, o, G=None):
+    """Gradient of mean((Y - T)^2) into G; returns (G, loss)."""
+    if G is None:
+        G = empty(P.numel(), "float32")
+    loss = C.c_double()
+    c = ctx()
+    c.check(lib().itg_mlp_backward(c.ptr, int(rows), int(i), int(h), int(o), ptr(P), ptr(X), ptr(T),
+                                   ptr(acts), ptr(Y), ptr(G), C.byref(loss)))
+    return G, loss.value
+
+
+def mlp_adam(P, G, m, v, step, lr=1e-3, b1=0.9, b2=0.999, eps=1e-8):
+    c = ctx()
+    c.check(lib().itg_mlp_adam(c.ptr, P.numel(), ptr(P), ptr(G), ptr(m), ptr(v), int(step),
+                               float(lr), float(b1), float(b2), float(eps)))
+    return P
+
+
+def mlp_features(r, diag, lam, chiL, d, chiR, floor=1e-8, row0=0, step=1, nrows=None, X=None):
+    """Feature rows (3*d*d floats each) of the residual r at Ritz value lam."""
+    if nrows is None:
+        nrows = (chiL * chiR - row0 + step - 1) // step
+    if X is None:
+        X = empty(nrows * 3 * d * d, "float32")
+    c = ctx()
+    c.check(lib().itg_mlp_features(c.ptr, chiL, d, chiR, ptr(r), ptr(diag), float(lam), float(floor),
+                                   int(row0), int(step), int(nrows), ptr(X)))
+    return X
+
+
+def mlp_gather(v, scale, chiL, d, chiR, row0=0, step=1, nrows=None, T=None):
+    if nrows is None:
+        nrows = (chiL * chiR - row0 + step - 1) // step
+    if T is None:
+        T = empty(nrows * d * d, "float32")
+    c = ctx()
+    c.check(lib().itg_mlp_gather(c.ptr, chiL, d, chiR, ptr(v), float(scale), int(row0), int(step),
+                                 int(nrows), ptr(T)))
+    return T
+
+
+def mlp_scatter(Y, chiL, d, chiR, row0, nrows, v):
+    c = ctx()
+    c.check(lib().itg_mlp_scatter(c.ptr, chiL, d, chiR, ptr(Y), int(row0), int(nrows), ptr(v)))
+    return v

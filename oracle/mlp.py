@@ -70,3 +70,44 @@ def adam(p, g, m, v, step, lr=1e-3, b1=0.9, b2=0.999, eps=1e-8):
     mh = m / (1 - b1 ** step)
     vh = v / (1 - b2 ** step)
     return p - lr * mh / (np.sqrt(vh) + eps), m, v
+
+
+# ------------------------------------------------ residual feature rows
+def _rows(chiL, chiR, row0, step, nrows):
+    if nrows is None:
+        nrows = (chiL * chiR - row0 + step - 1) // step
+    return row0 + step * np.arange(nrows)
+
+
+def _clip(den, floor):
+    return np.where(np.abs(den) < floor, np.where(den < 0, -floor, floor), den)
+
+
+def features(r, dg, lam, chiL, d, chiR, floor=1e-8, row0=0, step=1, nrows=None):
+    """Rows (a + chiL*b) of [r/rms(r), delta/rms(delta), (r/delta)/rms(r/delta)]
+    over each d*d physical block; delta = dg - lam clipped to |delta| >= floor."""
+    r = np.asarray(r, dtype=np.float64).reshape(chiL, d * d, chiR, order="F")
+    de = np.asarray(dg, dtype=np.float64).reshape(chiL, d * d, chiR, order="F") - lam
+    p = r / _clip(de, floor)
+
+    def inv_rms(x):
+        s = np.sqrt(np.mean(x * x))
+        return 1.0 / s if s > 0 else 1.0
+
+    rows = _rows(chiL, chiR, row0, step, nrows)
+    a, b = rows % chiL, rows // chiL
+    return np.concatenate([r[a, :, b] * inv_rms(r), de[a, :, b] * inv_rms(de),
+                           p[a, :, b] * inv_rms(p)], axis=1)
+
+
+def gather(v, scale, chiL, d, chiR, row0=0, step=1, nrows=None):
+    v = np.asarray(v, dtype=np.float64).reshape(chiL, d * d, chiR, order="F")
+    rows = _rows(chiL, chiR, row0, step, nrows)
+    return v[rows % chiL, :, rows // chiL] * scale
+
+
+def scatter(Y, chiL, d, chiR, row0, nrows, v):
+    v = np.asarray(v, dtype=np.float64).reshape(chiL, d * d, chiR, order="F").copy()
+    rows = row0 + np.arange(nrows)
+    v[rows % chiL, :, rows // chiL] = np.asarray(Y).reshape(nrows, d * d)
+    return v.ravel(order="F")

