@@ -5,9 +5,9 @@ Workload (the headline configuration): S=1/2 Heisenberg chain N=100, J=1,
 MPO bond dimension 5, two-site Davidson with 3 Krylov vectors, sweeps at
 maxdim 2048. A step is one full sweep (2(N-1) bond updates). Default start:
 a random right-canonical MPS with link dims min(2^j, 2^(N-j), 2048) (seed 1)
-and cutoff 1e-12 (random states have flat spectra, so nothing is truncated
-and every bulk bond update runs at the full chi=2048 shapes
-(Heff on 2048x2x2x2048, eigensolver n=4096). --config3 instead starts from
+and cutoff 1e-12: random states have flat spectra, so nothing is truncated
+and every bulk bond update runs at the full chi=2048 shapes (Heff on
+2048x2x2x2048, eigensolver n=4096). --config3 instead starts from
 link dims 10 with the maxdim ramp 64..1024 and cutoff 1e-12 (with which the
 bond dimension stays far below 2048 in the first sweeps). Both forms run
 from the same start: contractions in split-BF16 (3 products, FP32
@@ -45,11 +45,11 @@ def ramp_for(chi):
     return [m for m in (64, 128, 256, 512, 1024) if m < chi]
 
 
-def run_mode(mode, N, chi, steps, warmup, krylov, config3=False):
+def run_mode(mode, N, chi, steps, warmup, krylov, config3=False, timed_kinds=()):
     """Sweeps at maxdim chi. Default start: a random right-canonical MPS with
-    link dims min(2^j, 2^(N-j), chi) (seed 1), cutoff 0, so every bulk bond
-    runs at the full chi x chi shape. config3: random MPS with link dims 10
-    and the ramp 64..1024 before the chi sweeps, cutoff 1e-12."""
+    link dims min(2^j, 2^(N-j), chi) (seed 1); its flat spectra keep every
+    bulk bond at the full chi x chi shape under cutoff 1e-12. config3: random
+    MPS with link dims 10 and the ramp 64..1024 before the chi sweeps."""
     from itg import _lib
     from itg.mps import MPO, PhaseTimer, Sweeps, dmrg, heisenberg_opsum, random_mps
     import torch
@@ -80,6 +80,7 @@ def run_mode(mode, N, chi, steps, warmup, krylov, config3=False):
     rec = []
     torch.cuda.synchronize()
     _lib.ctx().sync()
+    _lib.ctx().timing(timed_kinds)
     t1 = time.perf_counter()
     E = dmrg(H, psi, Sweeps(steps, maxdim=[chi], cutoff=[cutoff]), mode=mode, krylovdim=krylov,
              ph=ph, timer=timer, record=rec,
@@ -87,8 +88,51 @@ def run_mode(mode, N, chi, steps, warmup, krylov, config3=False):
     torch.cuda.synchronize()
     _lib.ctx().sync()
     t2 = time.perf_counter()
+    kernels = {_lib.KERNEL_NAMES[k]: _lib.ctx().timing_report(k) for k in timed_kinds}
+    _lib.ctx().timing(())
     return {"s_per_sweep": (t2 - t1) / steps, "energy": E, "maxlinkdim": psi.maxlinkdim(),
-            "setup_s": setup_s, "phases": timer.report(), "sweeps": rec}
+            "setup_s": setup_s, "phases": timer.report(), "kernels": kernels, "sweeps": rec}
+
+
+def eigensolver_kernels(n):
+    """One density-matrix eigendecomposition at the sweep's size (n = d*chi,
+    maxdim n/2) with its tridiagonalisation and GEMM kernels event-timed."""
+    import torch
+    from itg import _lib, ops
+    A = ops.randn(n * n, 3)
+    rho = ops.empty(n * n)
+    ops.gemm("N", "T", n, n, n, 1.0 / n, A, n, A, n, 0.0, rho, n)
+    ops.eigh_trunc(ops.copy(rho), n, maxdim=n // 2, cutoff=1e-12)      # warm-up
+    kinds = (_lib.K_TRD_SYMV, _lib.K_TRD_POST, _lib.K_GETT)
+    r2 = ops.copy(rho)
+    torch.cuda.synchronize()
+    _lib.ctx().timing(kinds)
+    t0 = time.perf_counter()
+    ops.eigh_trunc(r2, n, maxdim=n // 2, cutoff=1e-12)
+    _lib.ctx().sync()
+    dt = time.perf_counter() - t0
+    out = {"n": n, "ms": dt * 1e3,
+           "kernels": {_lib.KERNEL_NAMES[k]: _lib.ctx().timing_report(k) for k in kinds}}
+    _lib.ctx().timing(())
+    return out
+
+
+def precision_check(N, chi, nsweep, krylov, modes):
+    """Every arithmetic form from the same start (random MPS, link dims 10,
+    seed 2) through nsweep sweeps with maxdim ramping to chi, cutoff 1e-12:
+    converged FP64 energies <psi|H|psi>, compared per site."""
+    from itg.mps import MPO, Sweeps, dmrg, heisenberg_opsum, random_mps
+    H = MPO.from_opsum(heisenberg_opsum([(j, j + 1) for j in range(N - 1)]), N)
+    ramp = [m for m in (16, 32, 64, 128, 256, 512) if m < chi] + [chi]
+    out = {}
+    for m in modes:
+        t0 = time.perf_counter()
+        psi = random_mps(N, 2, 10, seed=2)
+        out[m] = dmrg(H, psi, Sweeps(nsweep, maxdim=ramp, cutoff=[1e-12]), mode=m,
+                      krylovdim=krylov)
+        log("[%s] precision check: E = %.15f after %d sweeps (%.1f s)"
+            % (m, out[m], nsweep, time.perf_counter() - t0))
+    return out
 
 
 def cut_dims(N, d, chi):
@@ -139,13 +183,13 @@ def blas_threads():
         return 1
 
 
-def micro(reps=10):
-    """Heff apply of the chi=1024, w=5, d=2 microbench in each mode."""
+def micro(reps=10, chi=1024):
+    """Prepared Heff apply at chi (w=5, d=2) in each mode."""
     import numpy as np
     import torch
     from itg import ops
     from itg.mps import heff_flops
-    chi, w, d = 1024, 5, 2
+    w, d = 5, 2
     out = {}
     L = ops.randn(chi * w * chi, 11)
     R = ops.randn(chi * w * chi, 12)
@@ -154,15 +198,17 @@ def micro(reps=10):
     x = ops.randn(chi * d * d * chi, 15)
     y = ops.empty(chi * d * d * chi)
     work = ops.heff2_work(chi, w, w, w, d, chi)
+    dims = (chi, w, w, w, d, chi)
     for mode in ("native", "fp32", "bf16x3"):
+        prep = ops.heff2_prepare(L, W1, W2, R, *dims, mode=mode)
         for _ in range(2):
-            ops.heff2_apply(L, W1, W2, R, x, chi, w, w, w, d, chi, mode=mode, out=y, work=work)
+            ops.heff2_apply_prepared(L, R, prep, x, *dims, mode=mode, out=y, work=work)
         torch.cuda.synchronize()
         e0 = torch.cuda.Event(enable_timing=True)
         e1 = torch.cuda.Event(enable_timing=True)
         e0.record()
         for _ in range(reps):
-            ops.heff2_apply(L, W1, W2, R, x, chi, w, w, w, d, chi, mode=mode, out=y, work=work)
+            ops.heff2_apply_prepared(L, R, prep, x, *dims, mode=mode, out=y, work=work)
         e1.record()
         torch.cuda.synchronize()
         ms = e0.elapsed_time(e1) / reps
@@ -185,6 +231,9 @@ def main():
                     help="config-3 start: link dims 10, ramp 64..1024, cutoff 1e-12")
     ap.add_argument("--cpu-chi", type=int, default=1024)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--pc-chi", type=int, default=128,
+                    help="maxdim of the converged precision check (0 = skip)")
+    ap.add_argument("--pc-sweeps", type=int, default=10)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: the sweep's Heff/eigensolver path runs on a "
@@ -195,18 +244,42 @@ def main():
     info = _lib.ctx().info()
     log("device", info)
     if args.micro:
-        print(json.dumps({"micro": micro(), "device": info}))
+        print(json.dumps({"micro": {c: micro(chi=c) for c in (1024, 2048)}, "device": info}))
         return
     modes = args.modes.split(",")
+    from itg import _lib as L_
     res = {}
-    for m in modes:
-        res[m] = run_mode(m, args.N, args.chi, args.steps, args.warmup, args.krylov,
-                          args.config3)
-        log(m, {k: v for k, v in res[m].items() if k != "sweeps"})
     head = modes[0]
+    for m in modes:
+        kinds = ((L_.K_GEMM_SPLIT, L_.K_WPASS) if m == "bf16x3" else ()) if m == head else ()
+        res[m] = run_mode(m, args.N, args.chi, args.steps, args.warmup, args.krylov,
+                          args.config3, timed_kinds=kinds)
+        log(m, {k: v for k, v in res[m].items() if k != "sweeps"})
     r = res[head]
     heff = r["phases"].get("heff", {"ms": float("nan"), "flops": 0.0, "calls": 0})
-    achieved = heff["flops"] / (heff["ms"] / 1e3) / 1e12 if heff["ms"] > 0 else float("nan")
+    eig = eigensolver_kernels(2 * args.chi)
+    log("eigensolver", eig)
+    # dominant event-timed kernel of the headline form
+    kern = r.get("kernels") or {}
+    if kern:
+        kname, kd = max(kern.items(), key=lambda kv: kv[1]["ms"])
+        kms = kd["ms"] / max(kd["launches"], 1)
+        roof = {"bound": "mfma", "kernel": kname, "achieved": kd["flops"] / kd["ms"] / 1e9,
+                "peak": PEAK[head], "unit": "TFLOP/s", "traffic": None,
+                "launches": kd["launches"], "avg_ms": kms,
+                "flops_per_launch": kd["flops"] / max(kd["launches"], 1)}
+        roof["frac"] = roof["achieved"] / roof["peak"]
+    else:
+        achieved = heff["flops"] / (heff["ms"] / 1e3) / 1e12 if heff["ms"] > 0 else float("nan")
+        roof = {"bound": "mfma", "kernel": "two-site Heff apply (C1 + W pass + C4)",
+                "achieved": achieved, "peak": PEAK[head], "unit": "TFLOP/s",
+                "frac": achieved / PEAK[head], "traffic": None, "launches": heff["calls"],
+                "avg_ms": heff["ms"] / max(heff["calls"], 1)}
+    symv = eig["kernels"][L_.KERNEL_NAMES[L_.K_TRD_SYMV]]
+    roof_eig = {"bound": "hbm", "kernel": L_.KERNEL_NAMES[L_.K_TRD_SYMV],
+                "achieved": symv["bytes"] / symv["ms"] / 1e6, "peak": HBM_PEAK, "unit": "GB/s",
+                "launches": symv["launches"], "avg_ms": symv["ms"] / max(symv["launches"], 1)}
+    roof_eig["frac"] = roof_eig["achieved"] / HBM_PEAK
     line = {
         "metric": METRIC,
         "value": r["s_per_sweep"],
@@ -229,16 +302,27 @@ def main():
                                 "seed 1, cutoff 1e-12 (flat random spectra: bulk bonds stay at %d)" % (args.chi, args.chi)) +
                                "; one step = one sweep at maxdim %d" % args.chi,
                    "N": args.N, "chi": args.chi, "mpo_bond_dim": 5, "modes": modes},
-        "roofline": {"bound": "mfma", "kernel": "two-site Heff apply (GETT C1 + W pass + GETT C4)",
-                     "achieved": achieved, "peak": PEAK[head], "unit": "TFLOP/s",
-                     "frac": achieved / PEAK[head], "traffic": None,
-                     "launches": heff["calls"], "avg_ms": heff["ms"] / max(heff["calls"], 1)},
+        "roofline": roof,
+        "roofline_eigensolver": roof_eig,
+        "heff_apply": {"launches": heff["calls"], "avg_ms": heff["ms"] / max(heff["calls"], 1),
+                       "tflops": heff["flops"] / max(heff["ms"], 1e-9) / 1e9},
+        "eigensolver": eig,
         "results": {m: {k: v for k, v in res[m].items() if k != "sweeps"} for m in modes},
     }
     if len(modes) > 1:
         e = [res[m]["energy"] for m in modes]
-        line["dE_per_site"] = abs(e[0] - e[1]) / args.N
         line["speedup_vs_" + modes[1]] = res[modes[1]]["s_per_sweep"] / r["s_per_sweep"]
+        # after one sweep from a random state the two forms follow different
+        # Davidson trajectories, so this difference is not a precision figure
+        line["dE_per_site_timed_sweep"] = abs(e[0] - e[1]) / args.N
+        if args.pc_chi > 0:
+            pc = precision_check(args.N, args.pc_chi, args.pc_sweeps, args.krylov, modes)
+            line["dE_per_site"] = abs(pc[modes[0]] - pc[modes[1]]) / args.N
+            line["precision_check"] = {
+                "workload": "same chain, random MPS link dims 10 seed 2, %d sweeps, maxdim ramp "
+                            "to %d, cutoff 1e-12; FP64 energies of the converged states"
+                            % (args.pc_sweeps, args.pc_chi),
+                "energy": pc}
     if not args.no_cpu_baseline:
         cchi = min(args.cpu_chi, args.chi)
         t_bond = cpu_baseline(cchi)

@@ -68,6 +68,23 @@ void* itg_stream(itg_ctx* ctx);
 int itg_set_stream(itg_ctx* ctx, void* stream);
 /* Device properties: number of CUs and the gfx arch number (e.g. 950). */
 int itg_device_info(itg_ctx* ctx, int* num_cu, int* gfx_arch, double* hbm_gb);
+/* Per-kernel event timing on the context's stream (no counterpart in the
+ * reference; CUDA.jl's @elapsed in its benchmarks plays this role). While
+ * enabled, every launch of the kernel classes in `mask` (bit ITG_K_*) is
+ * bracketed by HIP events and credited with its algorithmic flops and
+ * bytes. A non-zero mask clears and starts, 0 stops. itg_timing_report
+ * synchronises and sums one class. */
+enum {
+  ITG_K_GEMM_SPLIT = 0,   /* split-BF16 GEMM (bf16x3 mode) */
+  ITG_K_GETT = 1,         /* contraction / GEMM kernel, every mode */
+  ITG_K_WPASS = 2,        /* Heff W1.W2 pass */
+  ITG_K_TRD_SYMV = 3,     /* tridiagonalisation: reflector + symmetric product */
+  ITG_K_TRD_POST = 4,     /* tridiagonalisation: w and next-column update */
+  ITG_K_COUNT = 5
+};
+int itg_timing(itg_ctx* ctx, int mask);
+int itg_timing_report(itg_ctx* ctx, int kind, int64_t* launches, double* ms,
+                      double* flops, double* bytes);
 
 /* ----------------------------------------------------------------- memory */
 /* Replace CuVector allocation / CuArray(x) / collect(x)
@@ -198,6 +215,23 @@ int itg_heff2_apply(itg_ctx* ctx, int dtype, int mode, int64_t chiL, int64_t wl,
                     int64_t wm, int64_t wr, int64_t d, int64_t chiR,
                     const void* L, const void* W1, const void* W2, const void* R,
                     const void* psi, void* out, void* work);
+/* The same product split in two for the Krylov loop at one bond: prepare
+ * once per (L, W1, W2, R) into `prep` (itg_heff2_prep_elems() 4-byte units:
+ * the fused W1.W2 operator and, in ITG_MODE_BF16X3, L and R split into bf16
+ * planes), then apply per vector with the same dtype, mode and dims. The
+ * ProjMPO caches its environments per bond the same way (ITensors' ProjMPO
+ * holds LR for the current position, reached through dmrg). */
+int64_t itg_heff2_prep_elems(int64_t chiL, int64_t wl, int64_t wm, int64_t wr,
+                             int64_t d, int64_t chiR);
+int itg_heff2_prepare(itg_ctx* ctx, int dtype, int mode, int64_t chiL, int64_t wl,
+                      int64_t wm, int64_t wr, int64_t d, int64_t chiR,
+                      const void* L, const void* W1, const void* W2, const void* R,
+                      void* prep);
+int itg_heff2_apply_prepared(itg_ctx* ctx, int dtype, int mode, int64_t chiL,
+                             int64_t wl, int64_t wm, int64_t wr, int64_t d,
+                             int64_t chiR, const void* L, const void* R,
+                             const void* prep, const void* psi, void* out,
+                             void* work);
 /* Diagonal of the same operator, shape (chiL,d,d,chiR). */
 int itg_heff2_diag(itg_ctx* ctx, int dtype, int64_t chiL, int64_t wl,
                    int64_t wm, int64_t wr, int64_t d, int64_t chiR,

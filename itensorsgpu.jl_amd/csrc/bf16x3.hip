@@ -1,0 +1,266 @@
+// Split-BF16 ("bf16x3") GEMM on operands split once into bf16 planes.
+//
+// x = hi + lo with hi = bf16_rne(x), lo = bf16_rne(x - hi); a product is
+// ah*bh + ah*bl + al*bh with FP32 accumulation on MFMA (lo*lo dropped).
+// A split operand is two planes, hi and lo, each row-major [row][k] with
+// leading dimension ldk (a multiple of 64, zero padded for k >= K), so an
+// MFMA lane's 8 consecutive k are one 16-byte load and the GEMM's inner loop
+// does no conversion and reads 4 bytes per element instead of 8.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdint>
+#include <type_traits>
+
+#include "bf16x3.h"
+#include "itg_internal.h"
+
+namespace {
+
+typedef __bf16 bf8_t __attribute__((ext_vector_type(8)));
+typedef float f4_t __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ uint16_t f2bf(float x) {
+  uint32_t u = __float_as_uint(x);
+  if ((u & 0x7f800000u) == 0x7f800000u) return (uint16_t)((u >> 16) | ((u & 0xffff) ? 0x40 : 0));
+  u += 0x7fffu + ((u >> 16) & 1u);   // round to nearest even
+  return (uint16_t)(u >> 16);
+}
+__device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+
+// ---------------------------------------------------------------- split
+// One block: a 64 (rows) x 64 (k) tile. Element (r, k) of the source is at
+// src[r*sr + k*sk]. Reads run along whichever of r, k has unit stride; the
+// tile is transposed through LDS when needed; writes are 16 bytes per lane.
+template <typename T>
+__global__ __launch_bounds__(256) void split_kernel(const T* __restrict__ src, int64_t R,
+                                                    int64_t K, int64_t sr, int64_t sk,
+                                                    int64_t ldk, int64_t tilesK,
+                                                    uint16_t* __restrict__ hi,
+                                                    uint16_t* __restrict__ lo) {
+  __shared__ float tile[64][65];
+  const int t = threadIdx.x;
+  const int64_t r0 = (blockIdx.x / tilesK) * 64;
+  const int64_t k0 = (blockIdx.x % tilesK) * 64;
+  if (sk == 1 || sr != 1) {
+    // k fastest: thread -> (k = t % 64, r = t / 64 + 4 i)
+    const int kk = t & 63;
+#pragma unroll 4
+    for (int i = 0; i < 16; ++i) {
+      int rr = (t >> 6) + 4 * i;
+      int64_t r = r0 + rr, k = k0 + kk;
+      tile[rr][kk] = (r < R && k < K) ? (float)src[r * sr + k * sk] : 0.0f;
+    }
+  } else {
+    // r fastest
+    const int rr = t & 63;
+#pragma unroll 4
+    for (int i = 0; i < 16; ++i) {
+      int kk = (t >> 6) + 4 * i;
+      int64_t r = r0 + rr, k = k0 + kk;
+      tile[rr][kk] = (r < R && k < K) ? (float)src[r * sr + k * sk] : 0.0f;
+    }
+  }
+  __syncthreads();
+  // 64 rows x 8 chunks of 8 k: two chunks per thread
+#pragma unroll
+  for (int i = 0; i < 2; ++i) {
+    int c = t + 256 * i;
+    int rr = c >> 3, kc = (c & 7) * 8;
+    int64_t r = r0 + rr;
+    if (r >= R) continue;
+    uint16_t h[8], l[8];
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      float x = tile[rr][kc + e];
+      h[e] = f2bf(x);
+      l[e] = f2bf(x - bf2f(h[e]));
+    }
+    uint4 hv, lv;
+    hv.x = h[0] | ((uint32_t)h[1] << 16); hv.y = h[2] | ((uint32_t)h[3] << 16);
+    hv.z = h[4] | ((uint32_t)h[5] << 16); hv.w = h[6] | ((uint32_t)h[7] << 16);
+    lv.x = l[0] | ((uint32_t)l[1] << 16); lv.y = l[2] | ((uint32_t)l[3] << 16);
+    lv.z = l[4] | ((uint32_t)l[5] << 16); lv.w = l[6] | ((uint32_t)l[7] << 16);
+    *(uint4*)(hi + r * ldk + k0 + kc) = hv;
+    *(uint4*)(lo + r * ldk + k0 + kc) = lv;
+  }
+}
+
+// ----------------------------------------------------------------- GEMM
+// C(m, n) = alpha * sum_k A(m, k) B(n, k) + beta * C(m, n), C column-major
+// with leading dimension ldc. 128 x 128 tile per 256-thread workgroup, 2 x 2
+// waves of 64 x 64, K-tile 64 (two 16x16x32 MFMA steps). The MFMA is issued
+// as D = B_frag . A_frag so that a lane's output column index runs along m:
+// sixteen consecutive m per store, C's unit-stride direction.
+constexpr int SB_M = 128, SB_N = 128, SB_K = 64, SB_PAD = 8, SB_LD = SB_K + SB_PAD;
+
+template <typename T>
+__global__ __launch_bounds__(256, 2) void gemm_split_kernel(
+    const uint16_t* __restrict__ Ah, const uint16_t* __restrict__ Al, int64_t lda,
+    const uint16_t* __restrict__ Bh, const uint16_t* __restrict__ Bl, int64_t ldb, int64_t M,
+    int64_t N, int64_t Kp, T* __restrict__ C, int64_t ldc, double alpha, double beta,
+    int64_t tilesM, int64_t tilesN) {
+  extern __shared__ __attribute__((aligned(16))) uint16_t lds[];
+  uint16_t* sAh = lds;
+  uint16_t* sAl = sAh + SB_M * SB_LD;
+  uint16_t* sBh = sAl + SB_M * SB_LD;
+  uint16_t* sBl = sBh + SB_N * SB_LD;
+
+  // XCD-aware grouped tile order (8 XCDs, groups of 8 tile rows)
+  int64_t nb = (int64_t)gridDim.x, bid = blockIdx.x;
+  int64_t lid = (nb % 8 == 0) ? (bid % 8) * (nb / 8) + bid / 8 : bid;
+  const int64_t GM = 8;
+  int64_t per_group = GM * tilesN;
+  int64_t group = lid / per_group;
+  int64_t first = group * GM;
+  int64_t gsize = std::min<int64_t>(tilesM - first, GM);
+  int64_t in = lid % per_group;
+  const int64_t m0 = (first + in % gsize) * SB_M;
+  const int64_t n0 = (in / gsize) * SB_N;
+
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  const int wm = wave & 1, wn = wave >> 1;
+  const int lr = lane & 15, lg = lane >> 4;
+
+  // loader: chunk c = t + 256 i (i < 4) -> row c / 8, k-chunk (c % 8) * 8
+  int64_t arow[4], brow[4];
+  int lrow[4], lkc[4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    int c = t + 256 * i;
+    lrow[i] = c >> 3;
+    lkc[i] = (c & 7) * 8;
+    arow[i] = std::min<int64_t>(m0 + lrow[i], M - 1) * lda + lkc[i];
+    brow[i] = std::min<int64_t>(n0 + lrow[i], N - 1) * ldb + lkc[i];
+  }
+  uint4 rah[4], ral[4], rbh[4], rbl[4];
+  auto load = [&](int64_t k0) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      rah[i] = *(const uint4*)(Ah + arow[i] + k0);
+      ral[i] = *(const uint4*)(Al + arow[i] + k0);
+      rbh[i] = *(const uint4*)(Bh + brow[i] + k0);
+      rbl[i] = *(const uint4*)(Bl + brow[i] + k0);
+    }
+  };
+  auto store = [&]() {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      int o = lrow[i] * SB_LD + lkc[i];
+      *(uint4*)(sAh + o) = rah[i];
+      *(uint4*)(sAl + o) = ral[i];
+      *(uint4*)(sBh + o) = rbh[i];
+      *(uint4*)(sBl + o) = rbl[i];
+    }
+  };
+
+  f4_t acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[i][j] = f4_t{0.f, 0.f, 0.f, 0.f};
+
+  load(0);
+  for (int64_t k0 = 0; k0 < Kp; k0 += SB_K) {
+    __syncthreads();
+    store();
+    __syncthreads();
+    if (k0 + SB_K < Kp) load(k0 + SB_K);
+#pragma unroll
+    for (int ks = 0; ks < SB_K; ks += 32) {
+      bf8_t ah[4], al[4], bh[4], bl[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        int ma = (wm * 64 + i * 16 + lr) * SB_LD + ks + lg * 8;
+        int nb_ = (wn * 64 + i * 16 + lr) * SB_LD + ks + lg * 8;
+        ah[i] = *(const bf8_t*)(sAh + ma);
+        al[i] = *(const bf8_t*)(sAl + ma);
+        bh[i] = *(const bf8_t*)(sBh + nb_);
+        bl[i] = *(const bf8_t*)(sBl + nb_);
+      }
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh[j], al[i], acc[i][j], 0, 0, 0);
+          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bl[j], ah[i], acc[i][j], 0, 0, 0);
+          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh[j], ah[i], acc[i][j], 0, 0, 0);
+        }
+    }
+  }
+
+  // lane holds D[n = 4 lg + r][m = lr] of block (i along m, j along n)
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    int64_t m = m0 + wm * 64 + i * 16 + lr;
+    if (m >= M) continue;
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        int64_t n = n0 + wn * 64 + j * 16 + 4 * lg + r;
+        if (n >= N) continue;
+        T* c = C + m + n * ldc;
+        double v = alpha * (double)acc[i][j][r];
+        if (beta != 0.0) v += beta * (double)*c;
+        *c = (T)v;
+      }
+  }
+}
+
+bool lds_ready = false;
+
+}  // namespace
+
+int64_t itg_split_ldk(int64_t K) { return (K + 63) / 64 * 64; }
+
+int itg_split_bf16(itg_ctx* ctx, int dtype, const void* src, int64_t R, int64_t K, int64_t sr,
+                   int64_t sk, int64_t ldk, uint16_t* hi, uint16_t* lo) {
+  if (R <= 0 || K <= 0) return ITG_OK;
+  if (ldk % 64 != 0 || ldk < K) return itg_fail(ctx, ITG_EARG, "split: ldk %lld", (long long)ldk);
+  int64_t tilesR = (R + 63) / 64, tilesK = ldk / 64;
+  int64_t blocks = tilesR * tilesK;
+  if (blocks > 0x7fffffff) return itg_fail(ctx, ITG_EUNSUPPORTED, "split: too large");
+  if (dtype == ITG_F64)
+    split_kernel<double><<<(unsigned)blocks, 256, 0, ctx->stream>>>(
+        (const double*)src, R, K, sr, sk, ldk, tilesK, hi, lo);
+  else
+    split_kernel<float><<<(unsigned)blocks, 256, 0, ctx->stream>>>(
+        (const float*)src, R, K, sr, sk, ldk, tilesK, hi, lo);
+  ITG_LAUNCH_CHECK(ctx);
+  return ITG_OK;
+}
+
+int itg_gemm_split(itg_ctx* ctx, int dtype, int64_t M, int64_t N, int64_t K, int64_t Kp,
+                   double alpha, const uint16_t* Ah, const uint16_t* Al, int64_t lda,
+                   const uint16_t* Bh, const uint16_t* Bl, int64_t ldb, double beta, void* C,
+                   int64_t ldc) {
+  if (M <= 0 || N <= 0) return ITG_OK;
+  if (Kp <= 0 || Kp % SB_K != 0 || lda < Kp || ldb < Kp || lda % 8 != 0 || ldb % 8 != 0)
+    return itg_fail(ctx, ITG_EARG, "split gemm: K %lld, lda %lld, ldb %lld", (long long)Kp,
+                    (long long)lda, (long long)ldb);
+  const size_t lds = (size_t)(2 * SB_M + 2 * SB_N) * SB_LD * sizeof(uint16_t);
+  if (!lds_ready) {
+    ITG_HIP(ctx, hipFuncSetAttribute((const void*)gemm_split_kernel<double>,
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    ITG_HIP(ctx, hipFuncSetAttribute((const void*)gemm_split_kernel<float>,
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    lds_ready = true;
+  }
+  int64_t tilesM = (M + SB_M - 1) / SB_M, tilesN = (N + SB_N - 1) / SB_N;
+  int64_t tiles = tilesM * tilesN;
+  if (tiles > 0x7fffffff) return itg_fail(ctx, ITG_EUNSUPPORTED, "split gemm: too large");
+  int tk = itg_tstart(ctx, ITG_K_GEMM_SPLIT);
+  if (dtype == ITG_F64)
+    gemm_split_kernel<double><<<(unsigned)tiles, 256, lds, ctx->stream>>>(
+        Ah, Al, lda, Bh, Bl, ldb, M, N, Kp, (double*)C, ldc, alpha, beta, tilesM, tilesN);
+  else
+    gemm_split_kernel<float><<<(unsigned)tiles, 256, lds, ctx->stream>>>(
+        Ah, Al, lda, Bh, Bl, ldb, M, N, Kp, (float*)C, ldc, alpha, beta, tilesM, tilesN);
+  ITG_LAUNCH_CHECK(ctx);
+  // algorithmic: 2MNK flops; split planes read once, C written (and read)
+  double es = (double)itg_dtype_size(dtype);
+  itg_tstop(ctx, tk, ITG_K_GEMM_SPLIT, 2.0 * M * N * K,
+            4.0 * (M + N) * K + es * M * N * (beta != 0.0 ? 2.0 : 1.0));
+  return ITG_OK;
+}

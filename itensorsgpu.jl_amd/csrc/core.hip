@@ -63,6 +63,7 @@ extern "C" int itg_destroy(itg_ctx* ctx) {
   hipSetDevice(ctx->device);
   hipStreamSynchronize(ctx->stream);
   itg_plan_cache_clear(ctx);
+  itg_timing_free(ctx);
   if (ctx->ws) hipFree(ctx->ws);
   if (ctx->eig_ws) hipFree(ctx->eig_ws);
   if (ctx->mlp_ws) hipFree(ctx->mlp_ws);
@@ -93,6 +94,75 @@ extern "C" int itg_device_info(itg_ctx* ctx, int* num_cu, int* gfx, double* hbm_
   if (num_cu) *num_cu = ctx->num_cu;
   if (gfx) *gfx = ctx->gfx;
   if (hbm_gb) *hbm_gb = ctx->hbm_gb;
+  return ITG_OK;
+}
+
+// ------------------------------------------------------------- timing
+static hipEvent_t timing_event(itg_ctx* ctx) {
+  if (ctx->event_next == ctx->event_pool.size()) {
+    hipEvent_t e;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    ctx->event_pool.push_back(e);
+  }
+  return ctx->event_pool[ctx->event_next++];
+}
+
+int itg_tstart(itg_ctx* ctx, int kind) {
+  if (!(ctx->timing & (1 << kind))) return -1;
+  hipEvent_t a = timing_event(ctx), b = timing_event(ctx);
+  if (!a || !b) return -1;
+  hipEventRecord(a, ctx->stream);
+  ctx->launches.push_back({-1, a, b, 0.0, 0.0});
+  return (int)ctx->launches.size() - 1;
+}
+
+void itg_tstop(itg_ctx* ctx, int t, int kind, double flops, double bytes) {
+  if (t < 0) return;
+  auto& L = ctx->launches[t];
+  hipEventRecord(L.b, ctx->stream);
+  L.kind = kind;
+  L.flops = flops;
+  L.bytes = bytes;
+}
+
+void itg_timing_free(itg_ctx* ctx) {
+  for (hipEvent_t e : ctx->event_pool) hipEventDestroy(e);
+  ctx->event_pool.clear();
+  ctx->launches.clear();
+  ctx->event_next = 0;
+}
+
+extern "C" int itg_timing(itg_ctx* ctx, int mask) {
+  ITG_CHECK_CTX(ctx);
+  if (mask) {
+    ITG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    ctx->launches.clear();
+    ctx->event_next = 0;
+  }
+  ctx->timing = mask & ((1 << ITG_K_COUNT) - 1);
+  return ITG_OK;
+}
+
+extern "C" int itg_timing_report(itg_ctx* ctx, int kind, int64_t* launches, double* ms,
+                                 double* flops, double* bytes) {
+  ITG_CHECK_CTX(ctx);
+  if (kind < 0 || kind >= ITG_K_COUNT) return itg_fail(ctx, ITG_EARG, "timing: kind %d", kind);
+  ITG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  int64_t n = 0;
+  double t = 0.0, f = 0.0, b = 0.0;
+  for (auto& L : ctx->launches) {
+    if (L.kind != kind) continue;
+    float e = 0.0f;
+    ITG_HIP(ctx, hipEventElapsedTime(&e, L.a, L.b));
+    ++n;
+    t += e;
+    f += L.flops;
+    b += L.bytes;
+  }
+  if (launches) *launches = n;
+  if (ms) *ms = t;
+  if (flops) *flops = f;
+  if (bytes) *bytes = b;
   return ITG_OK;
 }
 

@@ -97,10 +97,16 @@ __global__ __launch_bounds__(256) void trd_colnorm_kernel(const double* __restri
   if (threadIdx.x == 0) pnorm[blockIdx.x] = s;
 }
 
+// Waves [0, ceil(m/2)) take trailing columns (2w, 2w+1); the next 2i waves
+// take the panel vectors W(:,q), V(:,q). With VEC (n even, so every column
+// and the reflector start 16-byte aligned at even rows) each lane reads
+// double2 pairs of rows, four pairs in flight per column.
+template <bool VEC>
 __global__ __launch_bounds__(256) void trd_reflect_symv_kernel(
     const double* __restrict__ A, int n, int j, int i, const double* __restrict__ pnorm, int npn,
-    double* __restrict__ Vp, const double* __restrict__ Wp, double* __restrict__ y,
-    double* __restrict__ ab, double* __restrict__ pyv, double* d, double* e, double* tau) {
+    double* __restrict__ Vp, const double* __restrict__ Wp, double* __restrict__ V2,
+    double* __restrict__ y, double* __restrict__ ab, double* __restrict__ pyv, double* d,
+    double* e, double* tau) {
   __shared__ double sh4[4];
   __shared__ double shc[4];
   const size_t ld = (size_t)n;
@@ -116,6 +122,7 @@ __global__ __launch_bounds__(256) void trd_reflect_symv_kernel(
   int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   int gw = blockIdx.x * 4 + wv;
   int m = n - j - 1;
+  int npair = (m + 1) >> 1;
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     d[j] = a[j];
     e[j] = beta;
@@ -124,31 +131,75 @@ __global__ __launch_bounds__(256) void trd_reflect_symv_kernel(
   for (int r = blockIdx.x * 256 + threadIdx.x; r < n; r += gridDim.x * 256) {
     double v = r <= j ? 0.0 : (r == j + 1 ? 1.0 : a[r] * scale);
     Vp[r + (size_t)i * ld] = v;
+    V2[r + (size_t)i * ld] = v;
   }
   double contrib = 0.0;
-  if (gw < m) {
-    int c = j + 1 + gw;
-    const double* col = A + (size_t)c * ld;
-    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
-    int r = j + 2 + lane;
-    for (; r + 192 < n; r += 256) {
-      s0 += col[r] * a[r];
-      s1 += col[r + 64] * a[r + 64];
-      s2 += col[r + 128] * a[r + 128];
-      s3 += col[r + 192] * a[r + 192];
+  const int rs = j + 2;
+  if (gw < npair) {
+    const int c0 = j + 1 + 2 * gw;
+    const bool two = c0 + 1 < n;
+    const double* col0 = A + (size_t)c0 * ld;
+    const double* col1 = two ? col0 + ld : col0;
+    double s0 = 0.0, s1 = 0.0;
+    if (VEC) {
+      const int re = rs + (rs & 1);
+      if (lane == 0) {
+        if ((rs & 1) && rs < n) {
+          s0 += col0[rs] * a[rs];
+          s1 += col1[rs] * a[rs];
+        }
+        if (re < n && ((n - re) & 1)) {
+          s0 += col0[n - 1] * a[n - 1];
+          s1 += col1[n - 1] * a[n - 1];
+        }
+      }
+      const int rend = re + ((n - re) & ~1);   // rows [re, rend) in pairs
+      double t0 = 0.0, t1 = 0.0, u0 = 0.0, u1 = 0.0;
+      int r = re + 2 * lane;
+      for (; r + 384 < rend; r += 512) {
+        double2 x0 = *(const double2*)(a + r), x1 = *(const double2*)(a + r + 128);
+        double2 x2 = *(const double2*)(a + r + 256), x3 = *(const double2*)(a + r + 384);
+        double2 p0 = *(const double2*)(col0 + r), p1 = *(const double2*)(col0 + r + 128);
+        double2 p2 = *(const double2*)(col0 + r + 256), p3 = *(const double2*)(col0 + r + 384);
+        double2 q0 = *(const double2*)(col1 + r), q1 = *(const double2*)(col1 + r + 128);
+        double2 q2 = *(const double2*)(col1 + r + 256), q3 = *(const double2*)(col1 + r + 384);
+        t0 += p0.x * x0.x + p0.y * x0.y + p2.x * x2.x + p2.y * x2.y;
+        t1 += p1.x * x1.x + p1.y * x1.y + p3.x * x3.x + p3.y * x3.y;
+        u0 += q0.x * x0.x + q0.y * x0.y + q2.x * x2.x + q2.y * x2.y;
+        u1 += q1.x * x1.x + q1.y * x1.y + q3.x * x3.x + q3.y * x3.y;
+      }
+      for (; r < rend; r += 128) {
+        double2 x0 = *(const double2*)(a + r);
+        double2 p0 = *(const double2*)(col0 + r);
+        double2 q0 = *(const double2*)(col1 + r);
+        t0 += p0.x * x0.x + p0.y * x0.y;
+        u0 += q0.x * x0.x + q0.y * x0.y;
+      }
+      s0 += t0 + t1;
+      s1 += u0 + u1;
+    } else {
+      for (int r = rs + lane; r < n; r += 64) {
+        s0 += col0[r] * a[r];
+        s1 += col1[r] * a[r];
+      }
     }
-    for (; r < n; r += 64) s0 += col[r] * a[r];
-    double s = wave_sum((s0 + s1) + (s2 + s3));
-    double yc = col[j + 1] + scale * s;
+    s0 = wave_sum(s0);
+    s1 = wave_sum(s1);
     if (lane == 0) {
-      y[c] = yc;
-      contrib = yc * ((c == j + 1) ? 1.0 : a[c] * scale);
+      double y0 = col0[j + 1] + scale * s0;
+      y[c0] = y0;
+      contrib = y0 * ((c0 == j + 1) ? 1.0 : a[c0] * scale);
+      if (two) {
+        double y1 = col1[j + 1] + scale * s1;
+        y[c0 + 1] = y1;
+        contrib += y1 * a[c0 + 1] * scale;
+      }
     }
-  } else if (gw < m + 2 * i) {
-    int q = gw - m;
+  } else if (gw < npair + 2 * i) {
+    int q = gw - npair;
     const double* col = (q < i) ? Wp + (size_t)q * ld : Vp + (size_t)(q - i) * ld;
     double s = 0.0;
-    for (int r = j + 2 + lane; r < n; r += 64) s += col[r] * a[r];
+    for (int r = rs + lane; r < n; r += 64) s += col[r] * a[r];
     s = wave_sum(s);
     if (lane == 0) ab[q] = col[j + 1] + scale * s;
   }
@@ -556,8 +607,13 @@ static int eig_workspace(itg_ctx* ctx, size_t bytes, void** out) {
 // Tridiagonalise A (n x n FP64, full symmetric storage) in place: d, e out;
 // reflectors left in A's strictly lower part (v_j(j+1) = 1 explicit), tau out.
 static int tridiagonalize(itg_ctx* ctx, int n, double* A, double* d, double* e, double* tau,
-                          double* Vp, double* Wp, double* y, double* ab, double* pnorm,
-                          double* pyv) {
+                          double* VW, double* y, double* ab, double* pnorm, double* pyv) {
+  // VW = [V | W | V], n x NB each: the trailing update of a full panel is one
+  // K = 2 NB product A22 -= [V W] [W V]^T
+  double* Vp = VW;
+  double* Wp = VW + (size_t)n * NB;
+  double* V2 = VW + 2 * (size_t)n * NB;
+  const bool vec = (n % 2) == 0;
   for (int p = 0; p < n; p += NB) {
     int nbp = std::min(NB, n - p);
     const int npn = (n + TRD_ROWS - 1) / TRD_ROWS;
@@ -573,24 +629,39 @@ static int tridiagonalize(itg_ctx* ctx, int n, double* A, double* d, double* e, 
         break;
       }
       int m = n - j - 1;
-      int nsym = (m + 2 * i + 3) / 4;
-      trd_reflect_symv_kernel<<<nsym, 256, 0, ctx->stream>>>(A, n, j, i, pnorm, npn, Vp, Wp, y,
-                                                             ab, pyv, d, e, tau);
+      int nsym = ((m + 1) / 2 + 2 * i + 3) / 4;
+      int tk = itg_tstart(ctx, ITG_K_TRD_SYMV);
+      if (vec)
+        trd_reflect_symv_kernel<true><<<nsym, 256, 0, ctx->stream>>>(A, n, j, i, pnorm, npn, Vp,
+                                                                     Wp, V2, y, ab, pyv, d, e, tau);
+      else
+        trd_reflect_symv_kernel<false><<<nsym, 256, 0, ctx->stream>>>(A, n, j, i, pnorm, npn, Vp,
+                                                                      Wp, V2, y, ab, pyv, d, e, tau);
+      itg_tstop(ctx, tk, ITG_K_TRD_SYMV, 2.0 * m * (m + 2.0 * i), 8.0 * m * (m + 2.0 * i + 2));
       int next = (i + 1 < nbp) ? 1 : 0;
+      tk = itg_tstart(ctx, ITG_K_TRD_POST);
       trd_post_kernel<<<npn, 256, 0, ctx->stream>>>(A, n, j, i, nsym, Vp, Wp, y, ab, pyv, tau,
                                                     next, pnorm);
       ITG_LAUNCH_CHECK(ctx);
+      itg_tstop(ctx, tk, ITG_K_TRD_POST, 8.0 * m * (i + 1), 8.0 * m * (2.0 * i + 5));
     }
     int q = p + nbp;
     if (q < n) {
       int m2 = n - q;
       double* A22 = A + (size_t)q * n + q;
-      int rc = itg_gemm_internal(ctx, ITG_F64, ITG_MODE_NATIVE, 'N', 'T', m2, m2, nbp, -1.0,
-                                 Vp + q, n, Wp + q, n, 1.0, A22, n);
-      if (rc) return rc;
-      rc = itg_gemm_internal(ctx, ITG_F64, ITG_MODE_NATIVE, 'N', 'T', m2, m2, nbp, -1.0, Wp + q, n,
-                             Vp + q, n, 1.0, A22, n);
-      if (rc) return rc;
+      int rc;
+      if (nbp == NB) {
+        rc = itg_gemm_internal(ctx, ITG_F64, ITG_MODE_NATIVE, 'N', 'T', m2, m2, 2 * NB, -1.0,
+                               Vp + q, n, Wp + q, n, 1.0, A22, n);
+        if (rc) return rc;
+      } else {
+        rc = itg_gemm_internal(ctx, ITG_F64, ITG_MODE_NATIVE, 'N', 'T', m2, m2, nbp, -1.0, Vp + q,
+                               n, Wp + q, n, 1.0, A22, n);
+        if (rc) return rc;
+        rc = itg_gemm_internal(ctx, ITG_F64, ITG_MODE_NATIVE, 'N', 'T', m2, m2, nbp, -1.0, Wp + q,
+                               n, Vp + q, n, 1.0, A22, n);
+        if (rc) return rc;
+      }
     }
   }
   return ITG_OK;
@@ -709,7 +780,7 @@ extern "C" int itg_eigh_trunc(itg_ctx* ctx, int dtype, int64_t n64, void* Ain, d
   int n = (int)n64;
   size_t nn = (size_t)n * n;
   bool f32 = dtype == ITG_F32;
-  size_t bytes = (3 * nn + (f32 ? 2 * nn : 0)) * sizeof(double) + 2 * (size_t)n * NB * 8 +
+  size_t bytes = (3 * nn + (f32 ? 2 * nn : 0)) * sizeof(double) + 3 * (size_t)n * NB * 8 +
                  (size_t)n * 8 * 26 + 4 * (size_t)n * NB * 8 + 4 * NB * NB * 8 + 64 * 1024 +
                  64 * 256;
   void* ws;
@@ -721,8 +792,7 @@ extern "C" int itg_eigh_trunc(itg_ctx* ctx, int dtype, int64_t n64, void* Ain, d
   double* Vbuf = cv.take<double>(nn);
   double* A = f32 ? cv.take<double>(nn) : (double*)Ain;
   double* U = f32 ? cv.take<double>(nn) : (double*)Uout;
-  double* Vp = cv.take<double>((size_t)n * NB);
-  double* Wp = cv.take<double>((size_t)n * NB);
+  double* VW = cv.take<double>(3 * (size_t)n * NB);
   double* y = cv.take<double>(n);
   double* ab = cv.take<double>(2 * NB);
   double* pnorm = cv.take<double>((size_t)n / TRD_ROWS + 1);
@@ -751,7 +821,7 @@ extern "C" int itg_eigh_trunc(itg_ctx* ctx, int dtype, int64_t n64, void* Ain, d
     if (rc) return rc;
   }
   // 1. tridiagonalise
-  rc = tridiagonalize(ctx, n, A, d, e, tau, Vp, Wp, y, ab, pnorm, pyv);
+  rc = tridiagonalize(ctx, n, A, d, e, tau, VW, y, ab, pnorm, pyv);
   if (rc) return rc;
   // 2. divide and conquer
   double *Q, *D;

@@ -558,11 +558,14 @@ static int launch_gett_t(itg_ctx* ctx, GettArgs& g, bool ak, bool bk) {
     g.partial = ws;
   }
   dim3 grid((unsigned)tiles, 1, (unsigned)nsplit);
+  int tk = itg_tstart(ctx, ITG_K_GETT);
   if (ak && bk) gett_kernel<T, MODE, true, true><<<grid, 256, 0, ctx->stream>>>(g);
   else if (ak) gett_kernel<T, MODE, true, false><<<grid, 256, 0, ctx->stream>>>(g);
   else if (bk) gett_kernel<T, MODE, false, true><<<grid, 256, 0, ctx->stream>>>(g);
   else gett_kernel<T, MODE, false, false><<<grid, 256, 0, ctx->stream>>>(g);
   ITG_LAUNCH_CHECK(ctx);
+  itg_tstop(ctx, tk, ITG_K_GETT, 2.0 * g.M * g.N * g.K,
+            (double)sizeof(T) * (g.M * g.K + g.K * g.N + g.M * g.N * (g.beta != 0.0 ? 2 : 1)));
   if (nsplit > 1) {
     splitk_reduce_kernel<T, CT><<<itg_blocks(g.M * g.N, 256, 4096), 256, 0, ctx->stream>>>(g, nsplit);
     ITG_LAUNCH_CHECK(ctx);

@@ -10,8 +10,12 @@
 //
 // Algorithmic work at chiL = chiR = chi and wl = wr = w:
 // 4*w*d^2*chi^3 (C1 + C4) + 2*w^2*d^4*chi^2 (C23).
+// In bf16x3 mode L and R are split once per bond (itg_heff2_prepare) and psi
+// and T3 once per apply into k-contiguous bf16 planes, and C1/C4 run on the
+// split-operand GEMM (bf16x3.hip).
 #include <hip/hip_runtime.h>
 
+#include "bf16x3.h"
 #include "itg_internal.h"
 
 // X[q][o], q = wl + WL*(s1 + d*s2), o = s1' + d*(s2' + d*wr)
@@ -93,14 +97,186 @@ __global__ void heff_diag_kernel(int64_t chiL, int64_t WL, int64_t WM, int64_t W
   }
 }
 
+namespace {
+
+struct HeffDims {
+  int64_t chiL, wl, wm, wr, d, chiR, Q, O, M1, N1, K1, Kp1, M4, N4, K4, Kp4;
+  HeffDims(int64_t chiL_, int64_t wl_, int64_t wm_, int64_t wr_, int64_t d_, int64_t chiR_)
+      : chiL(chiL_), wl(wl_), wm(wm_), wr(wr_), d(d_), chiR(chiR_) {
+    Q = wl * d * d;
+    O = d * d * wr;
+    M1 = chiL * wl;
+    N1 = d * d * chiR;
+    K1 = chiL;
+    Kp1 = itg_split_ldk(K1);
+    M4 = chiL * d * d;
+    N4 = chiR;
+    K4 = wr * chiR;
+    Kp4 = itg_split_ldk(K4);
+  }
+};
+
+inline size_t al256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+// prep: X (Q x O doubles) | L planes (M1 x Kp1) | R planes (chiR x Kp4)
+struct Prep {
+  double* X;
+  uint16_t *Lh, *Ll, *Rh, *Rl;
+};
+size_t prep_bytes(const HeffDims& h) {
+  return al256(h.Q * h.O * sizeof(double)) + 2 * al256(h.M1 * h.Kp1 * 2) +
+         2 * al256(h.N4 * h.Kp4 * 2) + 256;
+}
+Prep carve_prep(const HeffDims& h, void* base) {
+  char* p = (char*)(((uintptr_t)base + 255) & ~(uintptr_t)255);
+  Prep r;
+  r.X = (double*)p;
+  p += al256(h.Q * h.O * sizeof(double));
+  r.Lh = (uint16_t*)p;
+  p += al256(h.M1 * h.Kp1 * 2);
+  r.Ll = (uint16_t*)p;
+  p += al256(h.M1 * h.Kp1 * 2);
+  r.Rh = (uint16_t*)p;
+  p += al256(h.N4 * h.Kp4 * 2);
+  r.Rl = (uint16_t*)p;
+  return r;
+}
+
+// work: T1 | T3 | psi planes (N1 x Kp1) | T3 planes (M4 x Kp4) | prep
+struct Work {
+  void *T1, *T3;
+  uint16_t *Ph, *Pl, *Th, *Tl;
+  void* prep;
+};
+size_t work_bytes(const HeffDims& h, size_t es) {
+  return al256(h.M1 * h.N1 * es) + al256(h.M4 * h.K4 * es) + 2 * al256(h.N1 * h.Kp1 * 2) +
+         2 * al256(h.M4 * h.Kp4 * 2) + prep_bytes(h) + 256;
+}
+Work carve_work(const HeffDims& h, size_t es, void* base) {
+  char* p = (char*)(((uintptr_t)base + 255) & ~(uintptr_t)255);
+  Work w;
+  w.T1 = p;
+  p += al256(h.M1 * h.N1 * es);
+  w.T3 = p;
+  p += al256(h.M4 * h.K4 * es);
+  w.Ph = (uint16_t*)p;
+  p += al256(h.N1 * h.Kp1 * 2);
+  w.Pl = (uint16_t*)p;
+  p += al256(h.N1 * h.Kp1 * 2);
+  w.Th = (uint16_t*)p;
+  p += al256(h.M4 * h.Kp4 * 2);
+  w.Tl = (uint16_t*)p;
+  p += al256(h.M4 * h.Kp4 * 2);
+  w.prep = p;
+  return w;
+}
+
+int check_heff(itg_ctx* ctx, int dtype, int mode, const HeffDims& h) {
+  ITG_CHECK_DTYPE(ctx, dtype);
+  if (mode < 0 || mode > 2) return itg_fail(ctx, ITG_EARG, "unknown mode %d", mode);
+  if (h.chiL <= 0 || h.chiR <= 0 || h.wl <= 0 || h.wm <= 0 || h.wr <= 0 || h.d <= 0)
+    return itg_fail(ctx, ITG_EDIM, "heff2: non-positive dimension");
+  if (h.Q * 64 + h.Q * h.O > 8192)   // LDS budget of the W pass (64 KB)
+    return itg_fail(ctx, ITG_EUNSUPPORTED, "heff2: MPO bond dimension too large for the W pass");
+  return ITG_OK;
+}
+
+int prepare(itg_ctx* ctx, int dtype, int mode, const HeffDims& h, const void* L, const void* W1,
+            const void* W2, const void* R, const Prep& P) {
+  if (dtype == ITG_F64)
+    heff_x_kernel<double><<<itg_blocks(h.Q * h.O, 256), 256, 0, ctx->stream>>>(
+        h.wl, h.wm, h.wr, h.d, (const double*)W1, (const double*)W2, P.X);
+  else
+    heff_x_kernel<float><<<itg_blocks(h.Q * h.O, 256), 256, 0, ctx->stream>>>(
+        h.wl, h.wm, h.wr, h.d, (const float*)W1, (const float*)W2, P.X);
+  ITG_LAUNCH_CHECK(ctx);
+  if (mode == ITG_MODE_BF16X3) {
+    int rc = itg_split_bf16(ctx, dtype, L, h.M1, h.K1, 1, h.M1, h.Kp1, P.Lh, P.Ll);
+    if (rc) return rc;
+    rc = itg_split_bf16(ctx, dtype, R, h.N4, h.K4, 1, h.chiR, h.Kp4, P.Rh, P.Rl);
+    if (rc) return rc;
+  }
+  return ITG_OK;
+}
+
+int apply(itg_ctx* ctx, int dtype, int mode, const HeffDims& h, const void* L, const void* R,
+          const Prep& P, const void* psi, void* out, const Work& w) {
+  int rc;
+  // C1: T1 = L(a'wl, a) . psi(a, s1 s2 b)
+  if (mode == ITG_MODE_BF16X3) {
+    rc = itg_split_bf16(ctx, dtype, psi, h.N1, h.K1, h.K1, 1, h.Kp1, w.Ph, w.Pl);
+    if (rc) return rc;
+    rc = itg_gemm_split(ctx, dtype, h.M1, h.N1, h.K1, h.Kp1, 1.0, P.Lh, P.Ll, h.Kp1, w.Ph, w.Pl,
+                        h.Kp1, 0.0, w.T1, h.M1);
+  } else {
+    rc = itg_gemm_internal(ctx, dtype, mode, 'N', 'N', h.M1, h.N1, h.K1, 1.0, L, h.M1, psi, h.K1,
+                           0.0, w.T1, h.M1);
+  }
+  if (rc) return rc;
+  // C23: W pass
+  int64_t blocks = ((h.chiL + 63) / 64) * h.chiR;
+  size_t lds = (size_t)(h.Q * 64 + h.Q * h.O) * sizeof(double);
+  int tk = itg_tstart(ctx, ITG_K_WPASS);
+  if (dtype == ITG_F64)
+    heff_wpass_kernel<double><<<(unsigned)blocks, 256, lds, ctx->stream>>>(
+        h.chiL, h.Q, h.O, (const double*)w.T1, P.X, (double*)w.T3);
+  else
+    heff_wpass_kernel<float><<<(unsigned)blocks, 256, lds, ctx->stream>>>(
+        h.chiL, h.Q, h.O, (const float*)w.T1, P.X, (float*)w.T3);
+  ITG_LAUNCH_CHECK(ctx);
+  itg_tstop(ctx, tk, ITG_K_WPASS, 2.0 * h.Q * h.O * h.chiL * h.chiR,
+            (double)(h.Q + h.O) * h.chiL * h.chiR * itg_dtype_size(dtype));
+  // C4: out(a's1's2', b') = T3(a's1's2', wr b) . R(b', wr b)^T
+  if (mode == ITG_MODE_BF16X3) {
+    rc = itg_split_bf16(ctx, dtype, w.T3, h.M4, h.K4, 1, h.M4, h.Kp4, w.Th, w.Tl);
+    if (rc) return rc;
+    return itg_gemm_split(ctx, dtype, h.M4, h.N4, h.K4, h.Kp4, 1.0, w.Th, w.Tl, h.Kp4, P.Rh,
+                          P.Rl, h.Kp4, 0.0, out, h.M4);
+  }
+  return itg_gemm_internal(ctx, dtype, mode, 'N', 'T', h.M4, h.N4, h.K4, 1.0, w.T3, h.M4, R,
+                           h.chiR, 0.0, out, h.M4);
+}
+
+}  // namespace
+
 extern "C" int64_t itg_heff2_work_elems(int64_t chiL, int64_t wl, int64_t wm, int64_t wr,
                                         int64_t d, int64_t chiR) {
-  (void)wm;
-  int64_t t1 = chiL * wl * d * d * chiR;
-  int64_t t3 = chiL * d * d * wr * chiR;
-  int64_t x = (wl * d * d) * (d * d * wr);
-  // X is stored as doubles: reserve 2 elements per entry for FP32 storage
-  return t1 + t3 + 2 * x + 64;
+  // counted in 4-byte units: enough for FP32 and FP64 buffers alike
+  HeffDims h(chiL, wl, wm, wr, d, chiR);
+  return (int64_t)((work_bytes(h, 8) + 3) / 4);
+}
+
+extern "C" int64_t itg_heff2_prep_elems(int64_t chiL, int64_t wl, int64_t wm, int64_t wr,
+                                        int64_t d, int64_t chiR) {
+  HeffDims h(chiL, wl, wm, wr, d, chiR);
+  return (int64_t)((prep_bytes(h) + 3) / 4);
+}
+
+extern "C" int itg_heff2_prepare(itg_ctx* ctx, int dtype, int mode, int64_t chiL, int64_t wl,
+                                 int64_t wm, int64_t wr, int64_t d, int64_t chiR, const void* L,
+                                 const void* W1, const void* W2, const void* R, void* prep) {
+  ITG_CHECK_CTX(ctx);
+  HeffDims h(chiL, wl, wm, wr, d, chiR);
+  int rc = check_heff(ctx, dtype, mode, h);
+  if (rc) return rc;
+  if (!L || !W1 || !W2 || !R || !prep) return itg_fail(ctx, ITG_EARG, "heff2: null pointer");
+  return prepare(ctx, dtype, mode, h, L, W1, W2, R, carve_prep(h, prep));
+}
+
+extern "C" int itg_heff2_apply_prepared(itg_ctx* ctx, int dtype, int mode, int64_t chiL,
+                                        int64_t wl, int64_t wm, int64_t wr, int64_t d,
+                                        int64_t chiR, const void* L, const void* R,
+                                        const void* prep, const void* psi, void* out,
+                                        void* work) {
+  ITG_CHECK_CTX(ctx);
+  HeffDims h(chiL, wl, wm, wr, d, chiR);
+  int rc = check_heff(ctx, dtype, mode, h);
+  if (rc) return rc;
+  if (!L || !R || !prep || !psi || !out || !work)
+    return itg_fail(ctx, ITG_EARG, "heff2: null pointer");
+  size_t es = itg_dtype_size(dtype);
+  return apply(ctx, dtype, mode, h, L, R, carve_prep(h, (void*)prep), psi, out,
+               carve_work(h, es, work));
 }
 
 extern "C" int itg_heff2_apply(itg_ctx* ctx, int dtype, int mode, int64_t chiL, int64_t wl,
@@ -108,46 +284,17 @@ extern "C" int itg_heff2_apply(itg_ctx* ctx, int dtype, int mode, int64_t chiL, 
                                const void* W1, const void* W2, const void* R, const void* psi,
                                void* out, void* work) {
   ITG_CHECK_CTX(ctx);
-  ITG_CHECK_DTYPE(ctx, dtype);
-  if (mode < 0 || mode > 2) return itg_fail(ctx, ITG_EARG, "unknown mode %d", mode);
-  if (chiL <= 0 || chiR <= 0 || wl <= 0 || wm <= 0 || wr <= 0 || d <= 0)
-    return itg_fail(ctx, ITG_EDIM, "heff2: non-positive dimension");
+  HeffDims h(chiL, wl, wm, wr, d, chiR);
+  int rc = check_heff(ctx, dtype, mode, h);
+  if (rc) return rc;
   if (!L || !W1 || !W2 || !R || !psi || !out || !work)
     return itg_fail(ctx, ITG_EARG, "heff2: null pointer");
   size_t es = itg_dtype_size(dtype);
-  int64_t Q = wl * d * d, O = d * d * wr;
-  if (Q * 64 + Q * O > 8192)   // LDS budget of the W pass (64 KB)
-    return itg_fail(ctx, ITG_EUNSUPPORTED, "heff2: MPO bond dimension too large for the W pass");
-  char* wk = (char*)work;
-  void* T1 = wk;
-  void* T3 = wk + (size_t)chiL * Q * chiR * es;
-  double* X = (double*)(wk + ((size_t)chiL * Q * chiR + (size_t)chiL * O * chiR) * es);
-  X = (double*)(((uintptr_t)X + 7) & ~(uintptr_t)7);
-  // X = W1.W2
-  if (dtype == ITG_F64)
-    heff_x_kernel<double><<<itg_blocks(Q * O, 256), 256, 0, ctx->stream>>>(
-        wl, wm, wr, d, (const double*)W1, (const double*)W2, X);
-  else
-    heff_x_kernel<float><<<itg_blocks(Q * O, 256), 256, 0, ctx->stream>>>(
-        wl, wm, wr, d, (const float*)W1, (const float*)W2, X);
-  ITG_LAUNCH_CHECK(ctx);
-  // C1: T1 = L(a'wl, a) . psi(a, s1 s2 b)
-  int rc = itg_gemm_internal(ctx, dtype, mode, 'N', 'N', chiL * wl, d * d * chiR, chiL, 1.0, L,
-                             chiL * wl, psi, chiL, 0.0, T1, chiL * wl);
+  Work w = carve_work(h, es, work);
+  Prep P = carve_prep(h, w.prep);
+  rc = prepare(ctx, dtype, mode, h, L, W1, W2, R, P);
   if (rc) return rc;
-  // C23: W pass
-  int64_t blocks = ((chiL + 63) / 64) * chiR;
-  size_t lds = (size_t)(Q * 64 + Q * O) * sizeof(double);
-  if (dtype == ITG_F64)
-    heff_wpass_kernel<double><<<(unsigned)blocks, 256, lds, ctx->stream>>>(
-        chiL, Q, O, (const double*)T1, X, (double*)T3);
-  else
-    heff_wpass_kernel<float><<<(unsigned)blocks, 256, lds, ctx->stream>>>(
-        chiL, Q, O, (const float*)T1, X, (float*)T3);
-  ITG_LAUNCH_CHECK(ctx);
-  // C4: out(a's1's2', b') = T3(a's1's2', wr b) . R(b', wr b)^T
-  return itg_gemm_internal(ctx, dtype, mode, 'N', 'T', chiL * d * d, chiR, wr * chiR, 1.0, T3,
-                           chiL * d * d, R, chiR, 0.0, out, chiL * d * d);
+  return apply(ctx, dtype, mode, h, L, R, P, psi, out, w);
 }
 
 extern "C" int itg_heff2_diag(itg_ctx* ctx, int dtype, int64_t chiL, int64_t wl, int64_t wm,

@@ -36,6 +36,16 @@ struct itg_ctx {
   // contraction plan cache (keyed by the canonical contraction signature)
   std::unordered_map<std::string, itg_plan*> plans;
   size_t plan_bytes = 0;
+  // per-kernel event timing (itg_timing)
+  int timing = 0;   // bit mask of timed kernel classes
+  struct TimedLaunch {
+    int kind;
+    hipEvent_t a, b;
+    double flops, bytes;
+  };
+  std::vector<TimedLaunch> launches;
+  std::vector<hipEvent_t> event_pool;
+  size_t event_next = 0;
 };
 
 constexpr int kRedSlots = 16384;
@@ -92,3 +102,9 @@ int itg_gemm_internal(itg_ctx* ctx, int dtype, int mode, char ta, char tb,
                       const void* A, int64_t lda, const void* B, int64_t ldb,
                       double beta, void* C, int64_t ldc);
 void itg_plan_cache_clear(itg_ctx* ctx);
+// Event bracket of one launch for itg_timing: t = itg_tstart(ctx, kind)
+// before the launch, itg_tstop(ctx, t, kind, flops, bytes) after it (no-ops
+// unless the kind is being timed).
+int itg_tstart(itg_ctx* ctx, int kind);
+void itg_tstop(itg_ctx* ctx, int t, int kind, double flops, double bytes);
+void itg_timing_free(itg_ctx* ctx);

@@ -19,6 +19,13 @@ F64, F32 = 0, 1
 MODE_NATIVE, MODE_F32, MODE_BF16X3 = 0, 1, 2
 TRUNC_NDTENSORS, TRUNC_REFGPU = 0, 1
 MODES = {"fp64": MODE_NATIVE, "native": MODE_NATIVE, "fp32": MODE_F32, "bf16x3": MODE_BF16X3}
+# kernel classes of itg_timing
+K_GEMM_SPLIT, K_GETT, K_WPASS, K_TRD_SYMV, K_TRD_POST = 0, 1, 2, 3, 4
+KERNEL_NAMES = {K_GEMM_SPLIT: "gemm_split_kernel (split-BF16 GEMM)",
+                K_GETT: "gett_kernel (contraction GEMM)",
+                K_WPASS: "heff_wpass_kernel (W1.W2 pass)",
+                K_TRD_SYMV: "trd_reflect_symv_kernel (tridiagonalisation symv)",
+                K_TRD_POST: "trd_post_kernel (tridiagonalisation update)"}
 
 
 class ITGError(RuntimeError):
@@ -48,6 +55,8 @@ _SIGS = {
     "itg_stream": (_vp, [_ctxp]),
     "itg_set_stream": (C.c_int, [_ctxp, _vp]),
     "itg_device_info": (C.c_int, [_ctxp, C.POINTER(C.c_int), C.POINTER(C.c_int), _dp]),
+    "itg_timing": (C.c_int, [_ctxp, C.c_int]),
+    "itg_timing_report": (C.c_int, [_ctxp, C.c_int, _i64p, _dp, _dp, _dp]),
     "itg_malloc": (C.c_int, [_ctxp, C.c_size_t, C.POINTER(_vp)]),
     "itg_free": (C.c_int, [_ctxp, _vp]),
     "itg_memcpy_h2d": (C.c_int, [_ctxp, _vp, _vp, C.c_size_t]),
@@ -86,6 +95,10 @@ _SIGS = {
     "itg_heff2_work_elems": (C.c_int64, [C.c_int64] * 6),
     "itg_heff2_apply": (C.c_int, [_ctxp, C.c_int, C.c_int] + [C.c_int64] * 6 + [_vp] * 7),
     "itg_heff2_diag": (C.c_int, [_ctxp, C.c_int] + [C.c_int64] * 6 + [_vp] * 5),
+    "itg_heff2_prep_elems": (C.c_int64, [C.c_int64] * 6),
+    "itg_heff2_prepare": (C.c_int, [_ctxp, C.c_int, C.c_int] + [C.c_int64] * 6 + [_vp] * 5),
+    "itg_heff2_apply_prepared": (C.c_int, [_ctxp, C.c_int, C.c_int] + [C.c_int64] * 6 +
+                                 [_vp] * 6),
     "itg_mlp_param_count": (C.c_int64, [C.c_int, C.c_int, C.c_int]),
     "itg_mlp_forward": (C.c_int, [_ctxp, C.c_int, C.c_int, C.c_int, C.c_int, _vp, _vp, _vp,
                                   _vp]),
@@ -167,6 +180,21 @@ class Context:
 
     def sync(self):
         self.check(lib().itg_sync(self.ptr))
+
+    def timing(self, kinds=()):
+        """Start (clearing) event timing of the kernel classes `kinds`
+        (K_* below); an empty set stops it (itg_timing)."""
+        mask = 0
+        for k in kinds:
+            mask |= 1 << int(k)
+        self.check(lib().itg_timing(self.ptr, mask))
+
+    def timing_report(self, kind):
+        """{launches, ms, flops, bytes} of one kernel class (K_* below)."""
+        n, ms, fl, by = C.c_int64(), C.c_double(), C.c_double(), C.c_double()
+        self.check(lib().itg_timing_report(self.ptr, int(kind), C.byref(n), C.byref(ms),
+                                           C.byref(fl), C.byref(by)))
+        return {"launches": n.value, "ms": ms.value, "flops": fl.value, "bytes": by.value}
 
 
 _ctx = None

@@ -306,6 +306,7 @@ class ProjMPO:
         N = len(H)
         self.L = [None] * (N + 1)
         self.R = [None] * (N + 1)
+        self._prep = None        # (bond, mode, dims, buffer) of itg_heff2_prepare
 
     def build_right(self, psi, upto=2):
         N = len(psi)
@@ -316,15 +317,37 @@ class ProjMPO:
                                      self.H.dims[j], self.env_mode)
         self.L[0] = ones_env()
 
+    def prepare(self, b, xdims):
+        """Fused W1.W2 operator (and, in bf16x3, the split L and R) for the
+        Krylov loop at bond b; product() uses it until the next prepare."""
+        L, Ld = self.L[b]
+        R, Rd = self.R[b + 2]
+        wl, wm = self.H.dims[b][0], self.H.dims[b][1]
+        wr = self.H.dims[b + 1][1]
+        chiL, d, _, chiR = xdims
+        dims = (chiL, wl, wm, wr, d, chiR)
+        buf = None
+        if self._prep is not None and self._prep[2] == dims:
+            buf = self._prep[3]
+        buf = ops.heff2_prepare(L, self.H.data[b], self.H.data[b + 1], R, *dims, mode=self.mode,
+                                prep=buf)
+        self._prep = (b, self.mode, dims, buf, L, R)
+
     def product(self, b, x, xdims, out=None, work=None):
         L, Ld = self.L[b]
         R, Rd = self.R[b + 2]
         wl, wm = self.H.dims[b][0], self.H.dims[b][1]
         wr = self.H.dims[b + 1][1]
         chiL, d, _, chiR = xdims
+        dims = (chiL, wl, wm, wr, d, chiR)
         ev = self.timer.start() if self.timer is not None else None
-        y = ops.heff2_apply(L, self.H.data[b], self.H.data[b + 1], R, x, chiL, wl, wm, wr, d,
-                            chiR, mode=self.mode, out=out, work=work)
+        p = self._prep
+        if (p is not None and p[0] == b and p[1] == self.mode and p[2] == dims and p[4] is L
+                and p[5] is R):
+            y = ops.heff2_apply_prepared(L, R, p[3], x, *dims, mode=self.mode, out=out, work=work)
+        else:
+            y = ops.heff2_apply(L, self.H.data[b], self.H.data[b + 1], R, x, *dims,
+                                mode=self.mode, out=out, work=work)
         if ev is not None:
             self.timer.stop("heff", ev, heff_flops(chiL, wl, wr, d, chiR))
         return y
@@ -538,6 +561,8 @@ def dmrg(H, psi, sweeps, mode="native", krylovdim=3, precond=None, errgoal=1e-14
             phi, pdims = ops.contract(A, Ad, [1, 2, 3], B, Bd, [3, 4, 5], [1, 2, 4, 5])
             work = ops.heff2_work(pdims[0], H.dims[b][0], H.dims[b][1], H.dims[b + 1][1],
                                   pdims[1], pdims[3], str(phi.dtype).split(".")[-1])
+
+            PH.prepare(b, pdims)
 
             def mv(x, b=b, pdims=pdims, work=work):
                 return PH.product(b, x, pdims, work=work)

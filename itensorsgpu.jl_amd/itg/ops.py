@@ -256,6 +256,34 @@ def heff2_apply(L, W1, W2, R, psi, chiL, wl, wm, wr, d, chiR, mode="native", out
     return out
 
 
+def heff2_prep(chiL, wl, wm, wr, d, chiR):
+    """Buffer for itg_heff2_prepare (counted in 4-byte units)."""
+    return empty(lib().itg_heff2_prep_elems(chiL, wl, wm, wr, d, chiR), "float32")
+
+
+def heff2_prepare(L, W1, W2, R, chiL, wl, wm, wr, d, chiR, mode="native", prep=None):
+    if prep is None:
+        prep = heff2_prep(chiL, wl, wm, wr, d, chiR)
+    c = ctx()
+    c.check(lib().itg_heff2_prepare(c.ptr, dtype_code(L), _lib.MODES[mode], chiL, wl, wm, wr, d,
+                                    chiR, ptr(L), ptr(W1), ptr(W2), ptr(R), ptr(prep)))
+    return prep
+
+
+def heff2_apply_prepared(L, R, prep, psi, chiL, wl, wm, wr, d, chiR, mode="native", out=None,
+                         work=None):
+    dt = str(psi.dtype).split(".")[-1]
+    if out is None:
+        out = empty(chiL * d * d * chiR, dt)
+    if work is None:
+        work = heff2_work(chiL, wl, wm, wr, d, chiR, dt)
+    c = ctx()
+    c.check(lib().itg_heff2_apply_prepared(c.ptr, dtype_code(psi), _lib.MODES[mode], chiL, wl, wm,
+                                           wr, d, chiR, ptr(L), ptr(R), ptr(prep), ptr(psi),
+                                           ptr(out), ptr(work)))
+    return out
+
+
 def heff2_diag(L, W1, W2, R, chiL, wl, wm, wr, d, chiR):
     dt = str(L.dtype).split(".")[-1]
     out = empty(chiL * d * d * chiR, dt)

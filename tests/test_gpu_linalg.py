@@ -57,6 +57,32 @@ def test_heff2_apply_matches_oracle(dev, chiL, wl, wm, wr, d, chiR):
                                odmrg.heff_diag(L, W1, W2, R), rtol=1e-12, atol=1e-12)
 
 
+@pytest.mark.parametrize("dtype", ["float64", "float32"])
+@pytest.mark.parametrize("mode", ["native", "fp32", "bf16x3"])
+def test_heff2_prepared_equals_one_shot(dev, dtype, mode):
+    """prepare once + apply per vector runs the same kernels as the one-shot
+    apply: bit-identical output, and within the mode's tolerance of the
+    oracle for FP64 and FP32 storage."""
+    rng = np.random.default_rng(5)
+    chiL, wl, wm, wr, d, chiR = 72, 5, 5, 5, 2, 136
+    L = sym_env(rng, chiL, wl)
+    R = sym_env(rng, chiR, wr)
+    W1 = rng.standard_normal((wl, wm, d, d))
+    W2 = rng.standard_normal((wm, wr, d, d))
+    dims = (chiL, wl, wm, wr, d, chiR)
+    args = [up(dev, a, dtype) for a in (L, W1, W2, R)]
+    prep = dev.heff2_prepare(*args, *dims, mode=mode)
+    tol = {"native": 1e-13 if dtype == "float64" else 2e-6, "fp32": 2e-5, "bf16x3": 5e-5}[mode]
+    for seed in (1, 2):
+        psi = np.random.default_rng(seed).standard_normal((chiL, d, d, chiR))
+        x = up(dev, psi, dtype)
+        y1 = dev.to_host(dev.heff2_apply(*args, x, *dims, mode=mode))
+        y2 = dev.to_host(dev.heff2_apply_prepared(args[0], args[3], prep, x, *dims, mode=mode))
+        assert np.array_equal(y1, y2)
+        want = odmrg.heff_apply(L, W1, W2, R, psi)
+        assert rel(ops.from_flat(y2, want.shape), want) < tol
+
+
 def test_heff2_config2_chi1024(dev):
     """Effective-Hamiltonian microbench shapes: chi=1024, w=5, d=2."""
     rng = np.random.default_rng(2)
