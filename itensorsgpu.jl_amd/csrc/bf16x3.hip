@@ -122,37 +122,41 @@ __global__ __launch_bounds__(256, 2) void gemm_split_kernel(
   const int wm = wave & 1, wn = wave >> 1;
   const int lr = lane & 15, lg = lane >> 4;
 
-  // loader: chunk c = t + 256 i (i < 4) -> row c / 8, k-chunk (c % 8) * 8
-  int64_t arow[4], brow[4];
-  int lrow[4], lkc[4];
-#pragma unroll
-  for (int i = 0; i < 4; ++i) {
-    int c = t + 256 * i;
-    lrow[i] = c >> 3;
-    lkc[i] = (c & 7) * 8;
-    arow[i] = std::min<int64_t>(m0 + lrow[i], M - 1) * lda + lkc[i];
-    brow[i] = std::min<int64_t>(n0 + lrow[i], N - 1) * ldb + lkc[i];
+  // loader: chunk c = t + 256 i (i < 4) -> row c / 8 = t / 8 + 32 i, k-chunk
+  // (t % 8) * 8; named registers (no arrays) for the prefetched chunks
+  const int lrow0 = t >> 3, lkc = (t & 7) * 8;
+  const int64_t ar0 = std::min<int64_t>(m0 + lrow0, M - 1) * lda + lkc;
+  const int64_t ar1 = std::min<int64_t>(m0 + lrow0 + 32, M - 1) * lda + lkc;
+  const int64_t ar2 = std::min<int64_t>(m0 + lrow0 + 64, M - 1) * lda + lkc;
+  const int64_t ar3 = std::min<int64_t>(m0 + lrow0 + 96, M - 1) * lda + lkc;
+  const int64_t br0 = std::min<int64_t>(n0 + lrow0, N - 1) * ldb + lkc;
+  const int64_t br1 = std::min<int64_t>(n0 + lrow0 + 32, N - 1) * ldb + lkc;
+  const int64_t br2 = std::min<int64_t>(n0 + lrow0 + 64, N - 1) * ldb + lkc;
+  const int64_t br3 = std::min<int64_t>(n0 + lrow0 + 96, N - 1) * ldb + lkc;
+  const int so = lrow0 * SB_LD + lkc, sstep = 32 * SB_LD;
+  uint4 ah0, ah1, ah2, ah3, al0, al1, al2, al3, bh0, bh1, bh2, bh3, bl0, bl1, bl2, bl3;
+#define SPLIT_LOAD(k0)                                                    \
+  {                                                                       \
+    ah0 = *(const uint4*)(Ah + ar0 + (k0)); al0 = *(const uint4*)(Al + ar0 + (k0)); \
+    ah1 = *(const uint4*)(Ah + ar1 + (k0)); al1 = *(const uint4*)(Al + ar1 + (k0)); \
+    ah2 = *(const uint4*)(Ah + ar2 + (k0)); al2 = *(const uint4*)(Al + ar2 + (k0)); \
+    ah3 = *(const uint4*)(Ah + ar3 + (k0)); al3 = *(const uint4*)(Al + ar3 + (k0)); \
+    bh0 = *(const uint4*)(Bh + br0 + (k0)); bl0 = *(const uint4*)(Bl + br0 + (k0)); \
+    bh1 = *(const uint4*)(Bh + br1 + (k0)); bl1 = *(const uint4*)(Bl + br1 + (k0)); \
+    bh2 = *(const uint4*)(Bh + br2 + (k0)); bl2 = *(const uint4*)(Bl + br2 + (k0)); \
+    bh3 = *(const uint4*)(Bh + br3 + (k0)); bl3 = *(const uint4*)(Bl + br3 + (k0)); \
   }
-  uint4 rah[4], ral[4], rbh[4], rbl[4];
-  auto load = [&](int64_t k0) {
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      rah[i] = *(const uint4*)(Ah + arow[i] + k0);
-      ral[i] = *(const uint4*)(Al + arow[i] + k0);
-      rbh[i] = *(const uint4*)(Bh + brow[i] + k0);
-      rbl[i] = *(const uint4*)(Bl + brow[i] + k0);
-    }
-  };
-  auto store = [&]() {
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      int o = lrow[i] * SB_LD + lkc[i];
-      *(uint4*)(sAh + o) = rah[i];
-      *(uint4*)(sAl + o) = ral[i];
-      *(uint4*)(sBh + o) = rbh[i];
-      *(uint4*)(sBl + o) = rbl[i];
-    }
-  };
+#define SPLIT_STORE()                                                     \
+  {                                                                       \
+    *(uint4*)(sAh + so) = ah0; *(uint4*)(sAl + so) = al0;                 \
+    *(uint4*)(sAh + so + sstep) = ah1; *(uint4*)(sAl + so + sstep) = al1; \
+    *(uint4*)(sAh + so + 2 * sstep) = ah2; *(uint4*)(sAl + so + 2 * sstep) = al2; \
+    *(uint4*)(sAh + so + 3 * sstep) = ah3; *(uint4*)(sAl + so + 3 * sstep) = al3; \
+    *(uint4*)(sBh + so) = bh0; *(uint4*)(sBl + so) = bl0;                 \
+    *(uint4*)(sBh + so + sstep) = bh1; *(uint4*)(sBl + so + sstep) = bl1; \
+    *(uint4*)(sBh + so + 2 * sstep) = bh2; *(uint4*)(sBl + so + 2 * sstep) = bl2; \
+    *(uint4*)(sBh + so + 3 * sstep) = bh3; *(uint4*)(sBl + so + 3 * sstep) = bl3; \
+  }
 
   f4_t acc[4][4];
 #pragma unroll
@@ -160,12 +164,12 @@ __global__ __launch_bounds__(256, 2) void gemm_split_kernel(
 #pragma unroll
     for (int j = 0; j < 4; ++j) acc[i][j] = f4_t{0.f, 0.f, 0.f, 0.f};
 
-  load(0);
+  SPLIT_LOAD(0);
   for (int64_t k0 = 0; k0 < Kp; k0 += SB_K) {
     __syncthreads();
-    store();
+    SPLIT_STORE();
     __syncthreads();
-    if (k0 + SB_K < Kp) load(k0 + SB_K);
+    if (k0 + SB_K < Kp) { SPLIT_LOAD(k0 + SB_K); }
 #pragma unroll
     for (int ks = 0; ks < SB_K; ks += 32) {
       bf8_t ah[4], al[4], bh[4], bl[4];

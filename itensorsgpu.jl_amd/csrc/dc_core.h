@@ -51,9 +51,14 @@ ITG_HD static inline double dc_secular_eval(int k, const double* dl, const doubl
   return f;
 }
 
-ITG_HD static inline void dc_secular_root(int i, int k, const double* dl, const double* w,
-                                          double rho, double w2sum, int* org_out,
-                                          double* tau_out) {
+// Root i of the secular equation, given `ev(o, tau, side, &dpo, &dpf, &ab)`
+// that returns f at x = dl[o] + tau with the derivative parts and the
+// absolute sum as in dc_secular_eval (serial on the host; the GPU kernel
+// passes a wave-parallel evaluation of the same sums).
+template <class Eval>
+ITG_HD static inline void dc_secular_root_with(int i, int k, const double* dl, double rho,
+                                               double w2sum, Eval&& ev, int* org_out,
+                                               double* tau_out) {
   int o, side;
   double lo, hi, D;
   if (i == k - 1) {
@@ -65,8 +70,8 @@ ITG_HD static inline void dc_secular_root(int i, int k, const double* dl, const 
   } else {
     double gap = dl[i + 1] - dl[i];
     double mid = 0.5 * gap;
-    double f = 1.0;
-    for (int j = 0; j < k; ++j) f += rho * w[j] * w[j] / ((dl[j] - dl[i]) - mid);
+    double d0, d1, d2;
+    double f = ev(i, mid, 1, &d0, &d1, &d2);
     if (f >= 0.0) {
       o = i; side = 1; lo = 0.0; hi = mid; D = gap;
     } else {
@@ -76,7 +81,7 @@ ITG_HD static inline void dc_secular_root(int i, int k, const double* dl, const 
   double tau = 0.5 * (lo + hi);
   for (int it = 0; it < 200; ++it) {
     double dpo, dpf, ab;
-    double f = dc_secular_eval(k, dl, w, rho, o, tau, side, &dpo, &dpf, &ab);
+    double f = ev(o, tau, side, &dpo, &dpf, &ab);
     if (f > 0.0) hi = tau; else lo = tau;
     if (fabs(f) <= 4.0 * ITG_DC_EPS * ab) break;
     if (hi - lo <= 2.0 * ITG_DC_EPS * fmax(fabs(lo), fabs(hi))) break;
@@ -84,10 +89,7 @@ ITG_HD static inline void dc_secular_root(int i, int k, const double* dl, const 
     double a = tau * tau * dpo;
     double xn;
     if (D == 0.0 || dpf == 0.0) {
-      double c = f + a / tau;            // f - a/(-tau)
-      if (D != 0.0) {
-        // far side empty: c + a/(-x) = 0 still models f
-      }
+      double c = f + a / tau;            // f - a/(-tau); far side empty
       xn = (c > 0.0) ? a / c : 0.5 * (lo + hi);
     } else {
       double b = (D - tau) * (D - tau) * dpf;
@@ -109,6 +111,17 @@ ITG_HD static inline void dc_secular_root(int i, int k, const double* dl, const 
   }
   *org_out = o;
   *tau_out = tau;
+}
+
+ITG_HD static inline void dc_secular_root(int i, int k, const double* dl, const double* w,
+                                          double rho, double w2sum, int* org_out,
+                                          double* tau_out) {
+  dc_secular_root_with(
+      i, k, dl, rho, w2sum,
+      [&](int o, double tau, int side, double* dpo, double* dpf, double* ab) {
+        return dc_secular_eval(k, dl, w, rho, o, tau, side, dpo, dpf, ab);
+      },
+      org_out, tau_out);
 }
 
 // zhat_i from the roots (x_j = dl[org_j] + tau_j):

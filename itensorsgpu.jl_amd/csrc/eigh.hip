@@ -97,10 +97,65 @@ __global__ __launch_bounds__(256) void trd_colnorm_kernel(const double* __restri
   if (threadIdx.x == 0) pnorm[blockIdx.x] = s;
 }
 
-// Waves [0, ceil(m/2)) take trailing columns (2w, 2w+1); the next 2i waves
-// take the panel vectors W(:,q), V(:,q). With VEC (n even, so every column
-// and the reflector start 16-byte aligned at even rows) each lane reads
-// double2 pairs of rows, four pairs in flight per column.
+// s0 = col0[rs:n] . a[rs:n], s1 = col1[rs:n] . a[rs:n], summed over the wave
+// (every lane gets the sums). With VEC the pointers are 16-byte aligned at
+// even rows and each lane reads double2 pairs, four pairs in flight.
+template <bool VEC>
+__device__ __forceinline__ void pair_dot(const double* __restrict__ col0,
+                                         const double* __restrict__ col1,
+                                         const double* __restrict__ a, int rs, int n, int lane,
+                                         double& s0, double& s1) {
+  s0 = 0.0;
+  s1 = 0.0;
+  if (VEC) {
+    const int re = rs + (rs & 1);
+    if (lane == 0) {
+      if ((rs & 1) && rs < n) {
+        s0 += col0[rs] * a[rs];
+        s1 += col1[rs] * a[rs];
+      }
+      if (re < n && ((n - re) & 1)) {
+        s0 += col0[n - 1] * a[n - 1];
+        s1 += col1[n - 1] * a[n - 1];
+      }
+    }
+    const int rend = re + ((n - re) & ~1);   // rows [re, rend) in pairs
+    double t0 = 0.0, t1 = 0.0, u0 = 0.0, u1 = 0.0;
+    int r = re + 2 * lane;
+    for (; r + 384 < rend; r += 512) {
+      double2 x0 = *(const double2*)(a + r), x1 = *(const double2*)(a + r + 128);
+      double2 x2 = *(const double2*)(a + r + 256), x3 = *(const double2*)(a + r + 384);
+      double2 p0 = *(const double2*)(col0 + r), p1 = *(const double2*)(col0 + r + 128);
+      double2 p2 = *(const double2*)(col0 + r + 256), p3 = *(const double2*)(col0 + r + 384);
+      double2 q0 = *(const double2*)(col1 + r), q1 = *(const double2*)(col1 + r + 128);
+      double2 q2 = *(const double2*)(col1 + r + 256), q3 = *(const double2*)(col1 + r + 384);
+      t0 += p0.x * x0.x + p0.y * x0.y + p2.x * x2.x + p2.y * x2.y;
+      t1 += p1.x * x1.x + p1.y * x1.y + p3.x * x3.x + p3.y * x3.y;
+      u0 += q0.x * x0.x + q0.y * x0.y + q2.x * x2.x + q2.y * x2.y;
+      u1 += q1.x * x1.x + q1.y * x1.y + q3.x * x3.x + q3.y * x3.y;
+    }
+    for (; r < rend; r += 128) {
+      double2 x0 = *(const double2*)(a + r);
+      double2 p0 = *(const double2*)(col0 + r);
+      double2 q0 = *(const double2*)(col1 + r);
+      t0 += p0.x * x0.x + p0.y * x0.y;
+      u0 += q0.x * x0.x + q0.y * x0.y;
+    }
+    s0 += t0 + t1;
+    s1 += u0 + u1;
+  } else {
+    for (int r = rs + lane; r < n; r += 64) {
+      s0 += col0[r] * a[r];
+      s1 += col1[r] * a[r];
+    }
+  }
+  s0 = wave_sum(s0);
+  s1 = wave_sum(s1);
+}
+
+// Waves [0, ceil(m/2)) take trailing columns (2w, 2w+1); the next i waves
+// take the panel vector pairs (W(:,q), V(:,q)). VEC needs n even, so every
+// column and the reflector start 16-byte aligned at even rows.
 template <bool VEC>
 __global__ __launch_bounds__(256) void trd_reflect_symv_kernel(
     const double* __restrict__ A, int n, int j, int i, const double* __restrict__ pnorm, int npn,
@@ -140,51 +195,8 @@ __global__ __launch_bounds__(256) void trd_reflect_symv_kernel(
     const bool two = c0 + 1 < n;
     const double* col0 = A + (size_t)c0 * ld;
     const double* col1 = two ? col0 + ld : col0;
-    double s0 = 0.0, s1 = 0.0;
-    if (VEC) {
-      const int re = rs + (rs & 1);
-      if (lane == 0) {
-        if ((rs & 1) && rs < n) {
-          s0 += col0[rs] * a[rs];
-          s1 += col1[rs] * a[rs];
-        }
-        if (re < n && ((n - re) & 1)) {
-          s0 += col0[n - 1] * a[n - 1];
-          s1 += col1[n - 1] * a[n - 1];
-        }
-      }
-      const int rend = re + ((n - re) & ~1);   // rows [re, rend) in pairs
-      double t0 = 0.0, t1 = 0.0, u0 = 0.0, u1 = 0.0;
-      int r = re + 2 * lane;
-      for (; r + 384 < rend; r += 512) {
-        double2 x0 = *(const double2*)(a + r), x1 = *(const double2*)(a + r + 128);
-        double2 x2 = *(const double2*)(a + r + 256), x3 = *(const double2*)(a + r + 384);
-        double2 p0 = *(const double2*)(col0 + r), p1 = *(const double2*)(col0 + r + 128);
-        double2 p2 = *(const double2*)(col0 + r + 256), p3 = *(const double2*)(col0 + r + 384);
-        double2 q0 = *(const double2*)(col1 + r), q1 = *(const double2*)(col1 + r + 128);
-        double2 q2 = *(const double2*)(col1 + r + 256), q3 = *(const double2*)(col1 + r + 384);
-        t0 += p0.x * x0.x + p0.y * x0.y + p2.x * x2.x + p2.y * x2.y;
-        t1 += p1.x * x1.x + p1.y * x1.y + p3.x * x3.x + p3.y * x3.y;
-        u0 += q0.x * x0.x + q0.y * x0.y + q2.x * x2.x + q2.y * x2.y;
-        u1 += q1.x * x1.x + q1.y * x1.y + q3.x * x3.x + q3.y * x3.y;
-      }
-      for (; r < rend; r += 128) {
-        double2 x0 = *(const double2*)(a + r);
-        double2 p0 = *(const double2*)(col0 + r);
-        double2 q0 = *(const double2*)(col1 + r);
-        t0 += p0.x * x0.x + p0.y * x0.y;
-        u0 += q0.x * x0.x + q0.y * x0.y;
-      }
-      s0 += t0 + t1;
-      s1 += u0 + u1;
-    } else {
-      for (int r = rs + lane; r < n; r += 64) {
-        s0 += col0[r] * a[r];
-        s1 += col1[r] * a[r];
-      }
-    }
-    s0 = wave_sum(s0);
-    s1 = wave_sum(s1);
+    double s0, s1;
+    pair_dot<VEC>(col0, col1, a, rs, n, lane, s0, s1);
     if (lane == 0) {
       double y0 = col0[j + 1] + scale * s0;
       y[c0] = y0;
@@ -195,13 +207,17 @@ __global__ __launch_bounds__(256) void trd_reflect_symv_kernel(
         contrib += y1 * a[c0 + 1] * scale;
       }
     }
-  } else if (gw < npair + 2 * i) {
+  } else if (gw < npair + i) {
+    // panel vectors W(:, q) and V(:, q) of one q, through the same loop
     int q = gw - npair;
-    const double* col = (q < i) ? Wp + (size_t)q * ld : Vp + (size_t)(q - i) * ld;
-    double s = 0.0;
-    for (int r = rs + lane; r < n; r += 64) s += col[r] * a[r];
-    s = wave_sum(s);
-    if (lane == 0) ab[q] = col[j + 1] + scale * s;
+    const double* col0 = Wp + (size_t)q * ld;
+    const double* col1 = Vp + (size_t)q * ld;
+    double s0, s1;
+    pair_dot<VEC>(col0, col1, a, rs, n, lane, s0, s1);
+    if (lane == 0) {
+      ab[q] = col0[j + 1] + scale * s0;
+      ab[i + q] = col1[j + 1] + scale * s1;
+    }
   }
   if (lane == 0) shc[wv] = contrib;
   __syncthreads();
@@ -303,27 +319,34 @@ __global__ void bt_build_v_kernel(const double* __restrict__ A, int n, int p, in
   }
 }
 
-// compact-WY triangular factor, forward columnwise (dlarft 'F','C')
+// compact-WY triangular factor, forward columnwise (dlarft 'F','C'); G and
+// tau are staged in LDS first so the column recurrence touches no HBM
 __global__ void bt_larft_kernel(int nbp, const double* __restrict__ G, const double* __restrict__ tau,
                                 double* __restrict__ T) {
   __shared__ double Ts[NB][NB + 1];
+  __shared__ double Gs[NB][NB + 1];
+  __shared__ double ts[NB];
   __shared__ double tmp[NB];
   int l = threadIdx.x;
-  for (int i = 0; i < NB; ++i) if (l < NB) Ts[l][i] = 0.0;
+  for (int e = l; e < NB * NB; e += blockDim.x) {
+    int r = e % NB, c = e / NB;
+    Ts[r][c] = 0.0;
+    Gs[r][c] = (r < nbp && c < nbp) ? G[r + c * nbp] : 0.0;
+  }
+  if (l < nbp) ts[l] = tau[l];
   __syncthreads();
   for (int i = 0; i < nbp; ++i) {
-    if (l < i) tmp[l] = -tau[i] * G[l + i * nbp];
+    if (l < i) tmp[l] = -ts[i] * Gs[l][i];
     __syncthreads();
     if (l < i) {
       double s = 0.0;
       for (int m = l; m < i; ++m) s += Ts[l][m] * tmp[m];
       Ts[l][i] = s;
     }
-    if (l == i) Ts[i][i] = tau[i];
+    if (l == i) Ts[i][i] = ts[i];
     __syncthreads();
   }
-  if (l < nbp)
-    for (int i = 0; i < nbp; ++i) T[l + i * nbp] = Ts[l][i];
+  for (int e = l; e < nbp * nbp; e += blockDim.x) T[e] = Ts[e % nbp][e / nbp];
 }
 
 // --------------------------------------------------------- divide & conquer
@@ -440,26 +463,71 @@ __global__ void dc_rotate_kernel(int n, int s_lvl, int half, double* __restrict_
   }
 }
 
-__global__ void dc_secular_kernel(int n, int s_lvl, int half, DcBufs b) {
+// One wave per root: lanes split the sums over the k poles (the xor-butterfly
+// sums leave every lane the same bits, so the iteration stays wave-uniform).
+__global__ __launch_bounds__(256) void dc_secular_kernel(int n, int s_lvl, int half, DcBufs b) {
   int m = blockIdx.x, r0, s;
   if (!merge_range(n, s_lvl, half, m, r0, s)) return;
   int k = b.meta[3 * m];
-  int i = blockIdx.y * blockDim.x + threadIdx.x;
+  int lane = threadIdx.x & 63;
+  int i = blockIdx.y * 4 + (threadIdx.x >> 6);
   if (i >= k) return;
+  const double* dl = b.dl + r0;
+  const double* w = b.wv + r0;
+  const double rho = b.mrho[2 * m];
+  auto ev = [&](int o, double tau, int side, double* dpo, double* dpf, double* absum) {
+    double f = 0.0, po = 0.0, pf = 0.0, ab = 0.0;
+    const double dlo = dl[o];
+    for (int j = lane; j < k; j += 64) {
+      double del = (dl[j] - dlo) - tau;
+      double t = w[j] / del;
+      double term = rho * w[j] * t;
+      f += term;
+      ab += fabs(term);
+      double dt = rho * t * t;
+      bool origin_side = side > 0 ? (j <= o) : (j >= o);
+      if (origin_side) po += dt; else pf += dt;
+    }
+    *dpo = wave_sum(po);
+    *dpf = wave_sum(pf);
+    *absum = 1.0 + wave_sum(ab);
+    return 1.0 + wave_sum(f);
+  };
   int o;
   double tv;
-  dc_secular_root(i, k, b.dl + r0, b.wv + r0, b.mrho[2 * m], b.mrho[2 * m + 1], &o, &tv);
-  b.org[r0 + i] = o;
-  b.tau[r0 + i] = tv;
+  dc_secular_root_with(i, k, dl, rho, b.mrho[2 * m + 1], ev, &o, &tv);
+  if (lane == 0) {
+    b.org[r0 + i] = o;
+    b.tau[r0 + i] = tv;
+  }
 }
 
-__global__ void dc_zhat_kernel(int n, int s_lvl, int half, DcBufs b) {
+__device__ __forceinline__ double wave_prod(double v) {
+  for (int o = 32; o > 0; o >>= 1) v *= __shfl_xor(v, o, 64);
+  return v;
+}
+
+// zhat_i (Gu-Eisenstat, as dc_zhat) with one wave per i
+__global__ __launch_bounds__(256) void dc_zhat_kernel(int n, int s_lvl, int half, DcBufs b) {
   int m = blockIdx.x, r0, s;
   if (!merge_range(n, s_lvl, half, m, r0, s)) return;
   int k = b.meta[3 * m];
-  int i = blockIdx.y * blockDim.x + threadIdx.x;
+  int lane = threadIdx.x & 63;
+  int i = blockIdx.y * 4 + (threadIdx.x >> 6);
   if (i >= k) return;
-  b.zh[r0 + i] = dc_zhat(i, k, b.dl + r0, b.org + r0, b.tau + r0, b.wv + r0);
+  const double* dl = b.dl + r0;
+  const int* org = b.org + r0;
+  const double* tau = b.tau + r0;
+  const double dli = dl[i];
+  double p = 1.0;
+  for (int j = lane; j < k; j += 64)
+    if (j != i) p *= ((dli - dl[org[j]]) - tau[j]) / (dli - dl[j]);
+  p = wave_prod(p) * ((dli - dl[org[i]]) - tau[i]);
+  if (lane == 0) {
+    double z2 = -p;
+    if (z2 < 0.0) z2 = 0.0;
+    b.zh[r0 + i] = copysign(sqrt(z2), b.wv[r0 + i]);
+  }
 }
 
 // column j of the rank-one eigenvector matrix, normalised (one wave per column)
@@ -629,7 +697,7 @@ static int tridiagonalize(itg_ctx* ctx, int n, double* A, double* d, double* e, 
         break;
       }
       int m = n - j - 1;
-      int nsym = ((m + 1) / 2 + 2 * i + 3) / 4;
+      int nsym = ((m + 1) / 2 + i + 3) / 4;
       int tk = itg_tstart(ctx, ITG_K_TRD_SYMV);
       if (vec)
         trd_reflect_symv_kernel<true><<<nsym, 256, 0, ctx->stream>>>(A, n, j, i, pnorm, npn, Vp,
@@ -688,8 +756,9 @@ static int tridiag_dc(itg_ctx* ctx, int n, const double* d, const double* e, dou
     ITG_HIP(ctx, hipStreamSynchronize(ctx->stream));
     dim3 g256(nm, (s_lvl + 255) / 256);
     dc_rotate_kernel<<<g256, 256, 0, ctx->stream>>>(n, s_lvl, half, Q, bufs);
-    dc_secular_kernel<<<g256, 256, 0, ctx->stream>>>(n, s_lvl, half, bufs);
-    dc_zhat_kernel<<<g256, 256, 0, ctx->stream>>>(n, s_lvl, half, bufs);
+    dim3 g4(nm, (s_lvl + 3) / 4);
+    dc_secular_kernel<<<g4, 256, 0, ctx->stream>>>(n, s_lvl, half, bufs);
+    dc_zhat_kernel<<<g4, 256, 0, ctx->stream>>>(n, s_lvl, half, bufs);
     dc_vec_kernel<<<dim3(nm, (s_lvl + 3) / 4), 256, 0, ctx->stream>>>(n, s_lvl, half, Vb, bufs);
     ITG_LAUNCH_CHECK(ctx);
     bool small = s_lvl <= SMALL_MERGE;

@@ -540,9 +540,10 @@ static int launch_gett_t(itg_ctx* ctx, GettArgs& g, bool ak, bool bk) {
   // split K when the output has too few tiles to fill the chip
   int nsplit = 1;
   int64_t target = 2 * (int64_t)std::max(ctx->num_cu, 1);
-  if (tiles < target && g.K >= 4 * TR::BKT * 16) {
+  if (tiles < target && g.K >= TR::BKT * 16) {
+    // at least 4 k-tiles per split
     int64_t want = (target + tiles - 1) / tiles;
-    int64_t maxs = g.K / (TR::BKT * 8);
+    int64_t maxs = g.K / (TR::BKT * 4);
     nsplit = (int)std::max<int64_t>(1, std::min<int64_t>({want, maxs, 64}));
   }
   g.partial = nullptr;
