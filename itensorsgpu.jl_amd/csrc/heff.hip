@@ -202,11 +202,14 @@ int prepare(itg_ctx* ctx, int dtype, int mode, const HeffDims& h, const void* L,
 int apply(itg_ctx* ctx, int dtype, int mode, const HeffDims& h, const void* L, const void* R,
           const Prep& P, const void* psi, void* out, const Work& w) {
   int rc;
+  const bool bf = mode == ITG_MODE_BF16X3;
+  // bf16x3 keeps T1 and T3 in FP32: the split operands carry 16 mantissa bits
+  const int idt = bf ? ITG_F32 : dtype;
   // C1: T1 = L(a'wl, a) . psi(a, s1 s2 b)
-  if (mode == ITG_MODE_BF16X3) {
+  if (bf) {
     rc = itg_split_bf16(ctx, dtype, psi, h.N1, h.K1, h.K1, 1, h.Kp1, w.Ph, w.Pl);
     if (rc) return rc;
-    rc = itg_gemm_split(ctx, dtype, h.M1, h.N1, h.K1, h.Kp1, 1.0, P.Lh, P.Ll, h.Kp1, w.Ph, w.Pl,
+    rc = itg_gemm_split(ctx, idt, h.M1, h.N1, h.K1, h.Kp1, 1.0, P.Lh, P.Ll, h.Kp1, w.Ph, w.Pl,
                         h.Kp1, 0.0, w.T1, h.M1);
   } else {
     rc = itg_gemm_internal(ctx, dtype, mode, 'N', 'N', h.M1, h.N1, h.K1, 1.0, L, h.M1, psi, h.K1,
@@ -217,7 +220,7 @@ int apply(itg_ctx* ctx, int dtype, int mode, const HeffDims& h, const void* L, c
   int64_t blocks = ((h.chiL + 63) / 64) * h.chiR;
   size_t lds = (size_t)(h.Q * 64 + h.Q * h.O) * sizeof(double);
   int tk = itg_tstart(ctx, ITG_K_WPASS);
-  if (dtype == ITG_F64)
+  if (idt == ITG_F64)
     heff_wpass_kernel<double><<<(unsigned)blocks, 256, lds, ctx->stream>>>(
         h.chiL, h.Q, h.O, (const double*)w.T1, P.X, (double*)w.T3);
   else
@@ -225,10 +228,10 @@ int apply(itg_ctx* ctx, int dtype, int mode, const HeffDims& h, const void* L, c
         h.chiL, h.Q, h.O, (const float*)w.T1, P.X, (float*)w.T3);
   ITG_LAUNCH_CHECK(ctx);
   itg_tstop(ctx, tk, ITG_K_WPASS, 2.0 * h.Q * h.O * h.chiL * h.chiR,
-            (double)(h.Q + h.O) * h.chiL * h.chiR * itg_dtype_size(dtype));
+            (double)(h.Q + h.O) * h.chiL * h.chiR * itg_dtype_size(idt));
   // C4: out(a's1's2', b') = T3(a's1's2', wr b) . R(b', wr b)^T
-  if (mode == ITG_MODE_BF16X3) {
-    rc = itg_split_bf16(ctx, dtype, w.T3, h.M4, h.K4, 1, h.M4, h.Kp4, w.Th, w.Tl);
+  if (bf) {
+    rc = itg_split_bf16(ctx, idt, w.T3, h.M4, h.K4, 1, h.M4, h.Kp4, w.Th, w.Tl);
     if (rc) return rc;
     return itg_gemm_split(ctx, dtype, h.M4, h.N4, h.K4, h.Kp4, 1.0, w.Th, w.Tl, h.Kp4, P.Rh,
                           P.Rl, h.Kp4, 0.0, out, h.M4);

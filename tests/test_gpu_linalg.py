@@ -191,6 +191,28 @@ def test_svd_invariants(dev, m, n):
     np.testing.assert_allclose(Sh, np.linalg.svd(A, compute_uv=False), rtol=1e-13)
 
 
+@pytest.mark.parametrize("m,n,r", [(300, 700, 50), (900, 400, 120)])
+def test_svd_rank_deficient_converges(dev, m, n, r):
+    """Columns that rotate down to the rounding level of ||A||_F must not keep
+    the one-sided Jacobi from converging (two-site tensors of DMRG are often
+    rank deficient); the rank-r part is reproduced and truncation at 1e-14
+    keeps exactly r values."""
+    rng = np.random.default_rng(m + n + r)
+    U0, _ = np.linalg.qr(rng.standard_normal((m, r)))
+    V0, _ = np.linalg.qr(rng.standard_normal((n, r)))
+    s = np.sort(rng.uniform(0.5, 2.0, r))[::-1]
+    A = (U0 * s) @ V0.T
+    U, S, V, k, te, dc = dev.svd_trunc(up(dev, A), m, n, cutoff=1e-14)
+    assert k == r
+    Uh = ops.from_flat(dev.to_host(U)[: m * k], (m, k))
+    Vh = ops.from_flat(dev.to_host(V)[: n * k], (n, k))
+    Sh = dev.to_host(S)[:k]
+    np.testing.assert_allclose(Sh, s, rtol=1e-12)
+    np.testing.assert_allclose(Uh @ np.diag(Sh) @ Vh.T, A, atol=1e-12)
+    assert np.abs(Uh.T @ Uh - np.eye(k)).max() < 1e-12
+    assert np.abs(Vh.T @ Vh - np.eye(k)).max() < 1e-12
+
+
 def test_svd_truncation(dev):
     rng = np.random.default_rng(3)
     m, n = 60, 50
