@@ -177,7 +177,7 @@ int itg_truncate_device(itg_ctx* ctx, const double* P_dev, int64_t n,
 
 /* -------------------------------------------------------- factorisations */
 /* Symmetric eigendecomposition A = U diag(w) U^T of a column-major n x n
- * matrix (lower triangle read). Eigenvalues are returned in DESCENDING order
+ * symmetric matrix (both triangles stored). Eigenvalues are returned in DESCENDING order
  * into `w` (device, FP64, length n) and the matching eigenvectors into the
  * columns of U (device, n x n, dtype). Then truncate! decides k
  * (src/tensor/culinearalgebra.jl:74-108: syevd, reverse, truncate!; mindim

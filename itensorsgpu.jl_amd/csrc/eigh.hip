@@ -97,75 +97,60 @@ __global__ __launch_bounds__(256) void trd_colnorm_kernel(const double* __restri
   if (threadIdx.x == 0) pnorm[blockIdx.x] = s;
 }
 
-// s0 = col0[rs:n] . a[rs:n], s1 = col1[rs:n] . a[rs:n], summed over the wave
-// (every lane gets the sums). With VEC the pointers are 16-byte aligned at
-// even rows and each lane reads double2 pairs, four pairs in flight.
-template <bool VEC>
-__device__ __forceinline__ void pair_dot(const double* __restrict__ col0,
-                                         const double* __restrict__ col1,
-                                         const double* __restrict__ a, int rs, int n, int lane,
-                                         double& s0, double& s1) {
-  s0 = 0.0;
-  s1 = 0.0;
-  if (VEC) {
-    const int re = rs + (rs & 1);
-    if (lane == 0) {
-      if ((rs & 1) && rs < n) {
-        s0 += col0[rs] * a[rs];
-        s1 += col1[rs] * a[rs];
-      }
-      if (re < n && ((n - re) & 1)) {
-        s0 += col0[n - 1] * a[n - 1];
-        s1 += col1[n - 1] * a[n - 1];
-      }
-    }
-    const int rend = re + ((n - re) & ~1);   // rows [re, rend) in pairs
-    double t0 = 0.0, t1 = 0.0, u0 = 0.0, u1 = 0.0;
-    int r = re + 2 * lane;
-    for (; r + 384 < rend; r += 512) {
-      double2 x0 = *(const double2*)(a + r), x1 = *(const double2*)(a + r + 128);
-      double2 x2 = *(const double2*)(a + r + 256), x3 = *(const double2*)(a + r + 384);
-      double2 p0 = *(const double2*)(col0 + r), p1 = *(const double2*)(col0 + r + 128);
-      double2 p2 = *(const double2*)(col0 + r + 256), p3 = *(const double2*)(col0 + r + 384);
-      double2 q0 = *(const double2*)(col1 + r), q1 = *(const double2*)(col1 + r + 128);
-      double2 q2 = *(const double2*)(col1 + r + 256), q3 = *(const double2*)(col1 + r + 384);
-      t0 += p0.x * x0.x + p0.y * x0.y + p2.x * x2.x + p2.y * x2.y;
-      t1 += p1.x * x1.x + p1.y * x1.y + p3.x * x3.x + p3.y * x3.y;
-      u0 += q0.x * x0.x + q0.y * x0.y + q2.x * x2.x + q2.y * x2.y;
-      u1 += q1.x * x1.x + q1.y * x1.y + q3.x * x3.x + q3.y * x3.y;
-    }
-    for (; r < rend; r += 128) {
-      double2 x0 = *(const double2*)(a + r);
-      double2 p0 = *(const double2*)(col0 + r);
-      double2 q0 = *(const double2*)(col1 + r);
-      t0 += p0.x * x0.x + p0.y * x0.y;
-      u0 += q0.x * x0.x + q0.y * x0.y;
-    }
-    s0 += t0 + t1;
-    s1 += u0 + u1;
-  } else {
-    for (int r = rs + lane; r < n; r += 64) {
-      s0 += col0[r] * a[r];
-      s1 += col1[r] * a[r];
-    }
-  }
-  s0 = wave_sum(s0);
-  s1 = wave_sum(s1);
-}
+// Symmetric product of the trailing matrix A22 (rows/cols o = j+1 .. n-1,
+// local index l = r - o, size m) with the reflector v, reading only the
+// 64 x 64 tiles on and below the diagonal (both triangles of a diagonal tile
+// are stored, so it is used whole). Tile (I, J), I >= J, writes its row
+// partial sum_{c in J} A[r,c] v[c] to Ypart[J][r] and, below the diagonal,
+// its column partial sum_{r in I} A[r,c] v[r] to Ypart[I][c]: every (t, l)
+// of Ypart (t < T tile rows) has exactly one writer and y(l) = sum_t
+// Ypart[t][l]. Its share of v^T A22 v goes to pyv[tile]. Blocks past the
+// tiles form the panel dots W(:,q).v and V(:,q).v. Every block forms the
+// reflector of column j from the norm partials (identically); block 0
+// stores d, e, tau; the grid writes v into the panel (and its mirror V2).
+constexpr int TS = 64;
 
-// Waves [0, ceil(m/2)) take trailing columns (2w, 2w+1); the next i waves
-// take the panel vector pairs (W(:,q), V(:,q)). VEC needs n even, so every
-// column and the reflector start 16-byte aligned at even rows.
-template <bool VEC>
-__global__ __launch_bounds__(256) void trd_reflect_symv_kernel(
+__global__ __launch_bounds__(256) void trd_symv_tiles_kernel(
     const double* __restrict__ A, int n, int j, int i, const double* __restrict__ pnorm, int npn,
     double* __restrict__ Vp, const double* __restrict__ Wp, double* __restrict__ V2,
-    double* __restrict__ y, double* __restrict__ ab, double* __restrict__ pyv, double* d,
-    double* e, double* tau) {
+    double* __restrict__ Ypart, double* __restrict__ ab, double* __restrict__ pyv, int ntiles,
+    double* d, double* e, double* tau) {
+  __shared__ double tile[TS][TS + 1];   // [column][row]
+  __shared__ double vr[TS], vc[TS];
+  __shared__ double red[4][TS];
   __shared__ double sh4[4];
-  __shared__ double shc[4];
   const size_t ld = (size_t)n;
   const double* a = A + (size_t)j * ld;
+  const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+  const int m = n - j - 1, o = j + 1;
+  const int bid = blockIdx.x;
+  // tile coordinates and the tile's loads first: they do not depend on the
+  // reflector, so their latency overlaps the norm reduction
+  int I = 0, J = 0;
+  double tv0 = 0, tv1 = 0, tv2 = 0, tv3 = 0, tv4 = 0, tv5 = 0, tv6 = 0, tv7 = 0, tv8 = 0, tv9 = 0,
+         tv10 = 0, tv11 = 0, tv12 = 0, tv13 = 0, tv14 = 0, tv15 = 0, araw = 0;
+  if (bid < ntiles) {
+    I = (int)((sqrt(8.0 * bid + 1.0) - 1.0) * 0.5);
+    while ((I + 1) * (I + 2) / 2 <= bid) ++I;
+    while (I * (I + 1) / 2 > bid) --I;
+    J = bid - I * (I + 1) / 2;
+    const int lr = TS * I + lane;
+    const bool rok = lr < m;
+    const double* src = A + (size_t)(o + (rok ? lr : 0)) + (size_t)o * ld;
+#define TLD(q) (rok && TS * J + wv + 4 * (q) < m ? src[(size_t)(TS * J + wv + 4 * (q)) * ld] : 0.0)
+    tv0 = TLD(0); tv1 = TLD(1); tv2 = TLD(2); tv3 = TLD(3);
+    tv4 = TLD(4); tv5 = TLD(5); tv6 = TLD(6); tv7 = TLD(7);
+    tv8 = TLD(8); tv9 = TLD(9); tv10 = TLD(10); tv11 = TLD(11);
+    tv12 = TLD(12); tv13 = TLD(13); tv14 = TLD(14); tv15 = TLD(15);
+#undef TLD
+    if (t < TS) {
+      int l = TS * I + t;
+      araw = l < m ? a[o + l] : 0.0;
+    } else if (t < 2 * TS) {
+      int l = TS * J + t - TS;
+      araw = l < m ? a[o + l] : 0.0;
+    }
+  }
   double xn2 = sum_partials(pnorm, npn, sh4);
   double alpha = a[j + 1];
   double beta = alpha, tv = 0.0, scale = 1.0;
@@ -174,54 +159,71 @@ __global__ __launch_bounds__(256) void trd_reflect_symv_kernel(
     tv = (beta - alpha) / beta;
     scale = 1.0 / (alpha - beta);
   }
-  int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  int gw = blockIdx.x * 4 + wv;
-  int m = n - j - 1;
-  int npair = (m + 1) >> 1;
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
+  if (blockIdx.x == 0 && t == 0) {
     d[j] = a[j];
     e[j] = beta;
     tau[j] = tv;
   }
-  for (int r = blockIdx.x * 256 + threadIdx.x; r < n; r += gridDim.x * 256) {
+  for (int r = blockIdx.x * 256 + t; r < n; r += gridDim.x * 256) {
     double v = r <= j ? 0.0 : (r == j + 1 ? 1.0 : a[r] * scale);
     Vp[r + (size_t)i * ld] = v;
     V2[r + (size_t)i * ld] = v;
   }
-  double contrib = 0.0;
-  const int rs = j + 2;
-  if (gw < npair) {
-    const int c0 = j + 1 + 2 * gw;
-    const bool two = c0 + 1 < n;
-    const double* col0 = A + (size_t)c0 * ld;
-    const double* col1 = two ? col0 + ld : col0;
-    double s0, s1;
-    pair_dot<VEC>(col0, col1, a, rs, n, lane, s0, s1);
-    if (lane == 0) {
-      double y0 = col0[j + 1] + scale * s0;
-      y[c0] = y0;
-      contrib = y0 * ((c0 == j + 1) ? 1.0 : a[c0] * scale);
-      if (two) {
-        double y1 = col1[j + 1] + scale * s1;
-        y[c0 + 1] = y1;
-        contrib += y1 * a[c0 + 1] * scale;
-      }
+  if (bid < ntiles) {
+    const int r0 = TS * I, c0 = TS * J;
+    tile[wv][lane] = tv0; tile[wv + 4][lane] = tv1; tile[wv + 8][lane] = tv2;
+    tile[wv + 12][lane] = tv3; tile[wv + 16][lane] = tv4; tile[wv + 20][lane] = tv5;
+    tile[wv + 24][lane] = tv6; tile[wv + 28][lane] = tv7; tile[wv + 32][lane] = tv8;
+    tile[wv + 36][lane] = tv9; tile[wv + 40][lane] = tv10; tile[wv + 44][lane] = tv11;
+    tile[wv + 48][lane] = tv12; tile[wv + 52][lane] = tv13; tile[wv + 56][lane] = tv14;
+    tile[wv + 60][lane] = tv15;
+    if (t < TS) {
+      int l = r0 + t;
+      vr[t] = l < m ? (l == 0 ? 1.0 : araw * scale) : 0.0;
+    } else if (t < 2 * TS) {
+      int l = c0 + t - TS;
+      vc[t - TS] = l < m ? (l == 0 ? 1.0 : araw * scale) : 0.0;
     }
-  } else if (gw < npair + i) {
-    // panel vectors W(:, q) and V(:, q) of one q, through the same loop
-    int q = gw - npair;
-    const double* col0 = Wp + (size_t)q * ld;
-    const double* col1 = Vp + (size_t)q * ld;
-    double s0, s1;
-    pair_dot<VEC>(col0, col1, a, rs, n, lane, s0, s1);
-    if (lane == 0) {
-      ab[q] = col0[j + 1] + scale * s0;
-      ab[i + q] = col1[j + 1] + scale * s1;
+    __syncthreads();
+    double s = 0.0;
+#pragma unroll 4
+    for (int cc = wv * 16; cc < wv * 16 + 16; ++cc) s += tile[cc][lane] * vc[cc];
+    red[wv][lane] = s;
+    __syncthreads();
+    double rowp = (red[0][lane] + red[1][lane]) + (red[2][lane] + red[3][lane]);
+    if (wv == 0 && r0 + lane < m) Ypart[(size_t)J * m + r0 + lane] = rowp;
+    if (I > J) {
+      double s2 = 0.0;
+#pragma unroll 4
+      for (int rr = wv * 16; rr < wv * 16 + 16; ++rr) s2 += tile[lane][rr] * vr[rr];
+      __syncthreads();   // every wave has read red
+      red[wv][lane] = s2;
+      __syncthreads();
+      if (wv == 0 && c0 + lane < m)
+        Ypart[(size_t)I * m + c0 + lane] =
+            (red[0][lane] + red[1][lane]) + (red[2][lane] + red[3][lane]);
+    }
+    if (wv == 0) {
+      double pv = wave_sum(rowp * vr[lane]);
+      if (lane == 0) pyv[bid] = (I > J ? 2.0 : 1.0) * pv;
+    }
+  } else {
+    const int q = bid - ntiles;   // q < i by the grid size
+    const double* wq = Wp + (size_t)q * ld + o;
+    const double* vq = Vp + (size_t)q * ld + o;
+    double s0 = 0.0, s1 = 0.0;
+    for (int l = t; l < m; l += 256) {
+      double vv = l == 0 ? 1.0 : a[o + l] * scale;
+      s0 += wq[l] * vv;
+      s1 += vq[l] * vv;
+    }
+    s0 = block_sum256(s0, sh4);
+    s1 = block_sum256(s1, sh4);
+    if (t == 0) {
+      ab[q] = s0;
+      ab[i + q] = s1;
     }
   }
-  if (lane == 0) shc[wv] = contrib;
-  __syncthreads();
-  if (threadIdx.x == 0) pyv[blockIdx.x] = (shc[0] + shc[1]) + (shc[2] + shc[3]);
 }
 
 // rows [64 b, 64 b + 64): w = tau (y - V a - W b) + alpha v, v -> A(:, j),
@@ -229,17 +231,30 @@ __global__ __launch_bounds__(256) void trd_reflect_symv_kernel(
 // four waves split the panel index q (q = wave mod 4) and meet in LDS once;
 // wave-level sums (identical in every wave) give y^T v, a^T b and w(j+1)
 // without block barriers.
-__device__ __forceinline__ double wave_sum_partials(const double* __restrict__ p, int np, int lane) {
-  double s = 0.0;
-  for (int b = lane; b < np; b += 64) s += p[b];
-  return wave_sum(s);
+// sum of p[b] over b = t (mod 256) for thread t, eight loads in flight
+__device__ __forceinline__ double strided_partial(const double* __restrict__ p, int np, int t) {
+  double q0 = 0.0, q1 = 0.0, q2 = 0.0, q3 = 0.0, q4 = 0.0, q5 = 0.0, q6 = 0.0, q7 = 0.0;
+  int b = t;
+  for (; b + 7 * 256 < np; b += 8 * 256) {
+    q0 += p[b];
+    q1 += p[b + 256];
+    q2 += p[b + 512];
+    q3 += p[b + 768];
+    q4 += p[b + 1024];
+    q5 += p[b + 1280];
+    q6 += p[b + 1536];
+    q7 += p[b + 1792];
+  }
+  for (; b < np; b += 256) q0 += p[b];
+  return ((q0 + q1) + (q2 + q3)) + ((q4 + q5) + (q6 + q7));
 }
+
 
 __global__ __launch_bounds__(256) void trd_post_kernel(
     double* __restrict__ A, int n, int j, int i, int nsym, const double* __restrict__ Vp,
-    double* __restrict__ Wp, const double* __restrict__ y, const double* __restrict__ ab,
-    const double* __restrict__ pyv, const double* __restrict__ tau, int next,
-    double* __restrict__ pnorm) {
+    double* __restrict__ Wp, const double* __restrict__ Ypart, int T,
+    const double* __restrict__ ab, const double* __restrict__ pyv,
+    const double* __restrict__ tau, int next, double* __restrict__ pnorm) {
   __shared__ double Vs[NB][TRD_ROWS + 1];
   __shared__ double Ws[NB][TRD_ROWS + 1];
   __shared__ double abs_[2 * NB];
@@ -253,12 +268,40 @@ __global__ __launch_bounds__(256) void trd_post_kernel(
   const int jn = j + 1;
   const double tj = tau[j];
   // per-row values of wave 0, loaded early
-  double yr = 0.0, anr = 0.0;
-  if (wv == 0 && r < n) {
-    yr = y[r];
-    if (next && r >= jn) anr = A[r + (size_t)jn * ld];
+  const int m = n - j - 1;
+  // y(r) = sum_t Ypart[t][r - jn]: wave w sums t = w (mod 4) for row `lane`;
+  // y(jn) with one t per lane, summed in every wave
+  __shared__ double ys[4][TRD_ROWS];
+  double yp = 0.0;
+  if (r > j && r < n) {
+    // eight independent loads in flight per lane
+    const double* yc = Ypart + (r - jn);
+    double q0 = 0.0, q1 = 0.0, q2 = 0.0, q3 = 0.0, q4 = 0.0, q5 = 0.0, q6 = 0.0, q7 = 0.0;
+    int tt = wv;
+    for (; tt + 28 < T; tt += 32) {
+      q0 += yc[(size_t)tt * m];
+      q1 += yc[(size_t)(tt + 4) * m];
+      q2 += yc[(size_t)(tt + 8) * m];
+      q3 += yc[(size_t)(tt + 12) * m];
+      q4 += yc[(size_t)(tt + 16) * m];
+      q5 += yc[(size_t)(tt + 20) * m];
+      q6 += yc[(size_t)(tt + 24) * m];
+      q7 += yc[(size_t)(tt + 28) * m];
+    }
+    for (; tt < T; tt += 4) q0 += yc[(size_t)tt * m];
+    yp = ((q0 + q1) + (q2 + q3)) + ((q4 + q5) + (q6 + q7));
   }
-  const double ytv = wave_sum_partials(pyv, nsym, lane);
+  ys[wv][lane] = yp;
+  double y0 = 0.0;
+  for (int tt = lane; tt < T; tt += 64) y0 += Ypart[(size_t)tt * m];
+  y0 = wave_sum(y0);
+  double anr = 0.0;
+  if (wv == 0 && r < n && next && r >= jn) anr = A[r + (size_t)jn * ld];
+  __shared__ double yv4[4];
+  {
+    double pv = wave_sum(strided_partial(pyv, nsym, t));
+    if (lane == 0) yv4[wv] = pv;
+  }
   if (t < 2 * i) abs_[t] = ab[t];
   // stage V(:, 0..i) and W(:, 0..i-1) for this block's rows
   for (int x = t; x < TRD_ROWS * (i + 1); x += 256) {
@@ -271,6 +314,7 @@ __global__ __launch_bounds__(256) void trd_post_kernel(
     xs[NB + t] = Wp[jn + (size_t)t * ld];
   }
   __syncthreads();
+  const double ytv = (yv4[0] + yv4[1]) + (yv4[2] + yv4[3]);
   // a^T b and the dot for w(jn): one term per lane, summed in each wave
   double atb = wave_sum(lane < i ? abs_[lane] * abs_[i + lane] : 0.0);
   const double alpha = -0.5 * tj * tj * (ytv - 2.0 * atb);
@@ -287,6 +331,7 @@ __global__ __launch_bounds__(256) void trd_post_kernel(
   pn[wv][lane] = sn;
   __syncthreads();
   if (wv != 0) return;
+  const double yr = (ys[0][lane] + ys[1][lane]) + (ys[2][lane] + ys[3][lane]);
   double s = 0.0;
   if (r < n) {
     double vr = Vs[i][lane];
@@ -299,7 +344,7 @@ __global__ __launch_bounds__(256) void trd_post_kernel(
     if (r > j) A[r + (size_t)j * ld] = vr;
     if (next && r >= jn) {
       // w(jn) for the panel column i (V(jn, i) = 1)
-      double wj1 = (tj != 0.0) ? tj * (y[jn] - djn) + alpha : 0.0;
+      double wj1 = (tj != 0.0) ? tj * (y0 - djn) + alpha : 0.0;
       double acc = anr - ((pn[0][lane] + pn[1][lane]) + (pn[2][lane] + pn[3][lane]));
       acc -= vr * wj1 + wr;
       A[r + (size_t)jn * ld] = acc;
@@ -362,6 +407,95 @@ __global__ void bt_larft_kernel(int nbp, const double* __restrict__ G, const dou
     __syncthreads();
   }
   for (int e = l; e < nbp * nbp; e += blockDim.x) T[e] = Ts[e % nbp][e / nbp];
+}
+
+// After the tridiagonalisation column j of A holds v_j below the diagonal
+// (v_j(j+1) = 1 stored); clearing rows 0..j makes A[:, p..p+NB) the panel's
+// reflector block V with no copy.
+__global__ void bt_clear_upper_kernel(double* __restrict__ A, int n) {
+  int64_t tot = (int64_t)n * n;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < tot;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    int r = (int)(e % n), j = (int)(e / n);
+    if (r <= j) A[e] = 0.0;
+  }
+}
+
+// All compact-WY factors at once: block b forms G = V_b^T V_b of panel b
+// (columns p = NB b .. p+nb, rows p+1..n-1, staged through LDS in 64-row
+// chunks) and runs the dlarft recurrence on it in LDS; T_b goes to
+// Tall[b * NB * NB] (nb x nb, column-major with leading dimension nb).
+__global__ __launch_bounds__(256) void bt_tall_kernel(const double* __restrict__ A, int n,
+                                                      int nrefl, const double* __restrict__ tau,
+                                                      double* __restrict__ Tall) {
+  __shared__ double Vs[64][NB + 1];
+  __shared__ double Gs[NB][NB + 1];
+  __shared__ double Ts[NB][NB + 1];
+  __shared__ double ts[NB];
+  __shared__ double tmp[NB];
+  const int t = threadIdx.x;
+  const int p = blockIdx.x * NB;
+  const int nb = min(NB, nrefl - p);
+  if (nb <= 0) return;
+  double g0 = 0.0, g1 = 0.0, g2 = 0.0, g3 = 0.0;   // pairs t, t+256, t+512, t+768
+  for (int r0 = p + 1; r0 < n; r0 += 64) {
+    __syncthreads();
+    for (int x = t; x < 64 * NB; x += 256) {
+      int rr = x % 64, c = x / 64, r = r0 + rr;
+      Vs[rr][c] = (r < n && c < nb) ? A[r + (size_t)(p + c) * n] : 0.0;
+    }
+    __syncthreads();
+    int rows = min(64, n - r0);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      int pr = t + 256 * q, ca = pr % NB, cb = pr / NB;
+      double s = 0.0;
+      for (int rr = 0; rr < rows; ++rr) s += Vs[rr][ca] * Vs[rr][cb];
+      if (q == 0) g0 += s; else if (q == 1) g1 += s; else if (q == 2) g2 += s; else g3 += s;
+    }
+  }
+  {
+    double gq[4] = {g0, g1, g2, g3};
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      int pr = t + 256 * q;
+      Gs[pr % NB][pr / NB] = gq[q];
+      Ts[pr % NB][pr / NB] = 0.0;
+    }
+  }
+  if (t < nb) ts[t] = tau[p + t];
+  __syncthreads();
+  for (int i = 0; i < nb; ++i) {
+    if (t < i) tmp[t] = -ts[i] * Gs[t][i];
+    __syncthreads();
+    if (t < i) {
+      double acc = 0.0;
+      for (int m = t; m < i; ++m) acc += Ts[t][m] * tmp[m];
+      Ts[t][i] = acc;
+    }
+    if (t == i) Ts[i][i] = ts[i];
+    __syncthreads();
+  }
+  double* T = Tall + (size_t)blockIdx.x * NB * NB;
+  for (int e = t; e < nb * nb; e += 256) T[e] = Ts[e % nb][e / nb];
+}
+
+// X2 = T X for one panel: T nb x nb upper triangular, X nb x k (ld nb);
+// one thread per element, eight columns per block (rows along lanes)
+__global__ __launch_bounds__(256) void bt_apply_t_kernel(int nb, int k, const double* __restrict__ T,
+                                                         const double* __restrict__ X,
+                                                         double* __restrict__ X2) {
+  __shared__ double Tsh[NB * NB];
+  __shared__ double Xs[8][NB];
+  const int t = threadIdx.x, r = t % NB, cl = t / NB;
+  const int c = blockIdx.x * 8 + cl;
+  for (int e = t; e < nb * nb; e += 256) Tsh[e] = T[e];
+  Xs[cl][r] = (c < k && r < nb) ? X[(size_t)c * nb + r] : 0.0;
+  __syncthreads();
+  if (c >= k || r >= nb) return;
+  double s = 0.0;
+  for (int q = r; q < nb; ++q) s += Tsh[r + q * nb] * Xs[cl][q];
+  X2[(size_t)c * nb + r] = s;
 }
 
 // --------------------------------------------------------- divide & conquer
@@ -690,13 +824,12 @@ static int eig_workspace(itg_ctx* ctx, size_t bytes, void** out) {
 // Tridiagonalise A (n x n FP64, full symmetric storage) in place: d, e out;
 // reflectors left in A's strictly lower part (v_j(j+1) = 1 explicit), tau out.
 static int tridiagonalize(itg_ctx* ctx, int n, double* A, double* d, double* e, double* tau,
-                          double* VW, double* y, double* ab, double* pnorm, double* pyv) {
+                          double* VW, double* Ypart, double* ab, double* pnorm, double* pyv) {
   // VW = [V | W | V], n x NB each: the trailing update of a full panel is one
   // K = 2 NB product A22 -= [V W] [W V]^T
   double* Vp = VW;
   double* Wp = VW + (size_t)n * NB;
   double* V2 = VW + 2 * (size_t)n * NB;
-  const bool vec = (n % 2) == 0;
   for (int p = 0; p < n; p += NB) {
     int nbp = std::min(NB, n - p);
     const int npn = (n + TRD_ROWS - 1) / TRD_ROWS;
@@ -712,21 +845,22 @@ static int tridiagonalize(itg_ctx* ctx, int n, double* A, double* d, double* e, 
         break;
       }
       int m = n - j - 1;
-      int nsym = ((m + 1) / 2 + i + 3) / 4;
+      int T = (m + TS - 1) / TS;
+      int ntiles = T * (T + 1) / 2;
       int tk = itg_tstart(ctx, ITG_K_TRD_SYMV);
-      if (vec)
-        trd_reflect_symv_kernel<true><<<nsym, 256, 0, ctx->stream>>>(A, n, j, i, pnorm, npn, Vp,
-                                                                     Wp, V2, y, ab, pyv, d, e, tau);
-      else
-        trd_reflect_symv_kernel<false><<<nsym, 256, 0, ctx->stream>>>(A, n, j, i, pnorm, npn, Vp,
-                                                                      Wp, V2, y, ab, pyv, d, e, tau);
-      itg_tstop(ctx, tk, ITG_K_TRD_SYMV, 2.0 * m * (m + 2.0 * i), 8.0 * m * (m + 2.0 * i + 2));
+      trd_symv_tiles_kernel<<<ntiles + i, 256, 0, ctx->stream>>>(A, n, j, i, pnorm, npn, Vp, Wp,
+                                                                 V2, Ypart, ab, pyv, ntiles, d, e,
+                                                                 tau);
+      ITG_LAUNCH_CHECK(ctx);
+      // lower triangle of A22 read once, the panel pairs over m rows
+      itg_tstop(ctx, tk, ITG_K_TRD_SYMV, 2.0 * m * (m + 2.0 * i),
+                8.0 * (0.5 * m * (m + TS) + 2.0 * i * m + 2.0 * T * m));
       int next = (i + 1 < nbp) ? 1 : 0;
       tk = itg_tstart(ctx, ITG_K_TRD_POST);
-      trd_post_kernel<<<npn, 256, 0, ctx->stream>>>(A, n, j, i, nsym, Vp, Wp, y, ab, pyv, tau,
-                                                    next, pnorm);
+      trd_post_kernel<<<npn, 256, 0, ctx->stream>>>(A, n, j, i, ntiles, Vp, Wp, Ypart, T, ab, pyv,
+                                                    tau, next, pnorm);
       ITG_LAUNCH_CHECK(ctx);
-      itg_tstop(ctx, tk, ITG_K_TRD_POST, 8.0 * m * (i + 1), 8.0 * m * (2.0 * i + 5));
+      itg_tstop(ctx, tk, ITG_K_TRD_POST, 8.0 * m * (i + 1), 8.0 * m * (2.0 * i + 5 + T));
     }
     int q = p + nbp;
     if (q < n) {
@@ -845,6 +979,36 @@ static int back_transform(itg_ctx* ctx, int n, int nrefl, int off, int k, const 
   return ITG_OK;
 }
 
+// U (n x k) <- Q U for the tridiagonalisation's Q = H_0 ... H_{n-2}, with
+// the reflectors read in place from A (after bt_clear_upper) and every T
+// formed up front by bt_tall_kernel; per panel X = V^T U, X2 = T X,
+// U -= V X2.
+static int back_transform_trd(itg_ctx* ctx, int n, int k, double* A, const double* tau,
+                              double* U, double* Tall, double* X, double* X2) {
+  int nrefl = n - 1;
+  if (nrefl < 1 || k < 1) return ITG_OK;
+  bt_clear_upper_kernel<<<itg_blocks((int64_t)n * n, 256, 16384), 256, 0, ctx->stream>>>(A, n);
+  int npan = (nrefl + NB - 1) / NB;
+  bt_tall_kernel<<<npan, 256, 0, ctx->stream>>>(A, n, nrefl, tau, Tall);
+  ITG_LAUNCH_CHECK(ctx);
+  for (int b = npan - 1; b >= 0; --b) {
+    int p = b * NB;
+    int nbp = std::min(NB, nrefl - p);
+    int m2 = n - p - 1;
+    const double* V1 = A + (size_t)p * n + (p + 1);   // rows p+1.., columns p..p+nbp
+    int rc = itg_gemm_internal(ctx, ITG_F64, ITG_MODE_NATIVE, 'T', 'N', nbp, k, m2, 1.0, V1, n,
+                               U + (p + 1), n, 0.0, X, nbp);
+    if (rc) return rc;
+    bt_apply_t_kernel<<<(k + 7) / 8, 256, 0, ctx->stream>>>(nbp, k, Tall + (size_t)b * NB * NB, X,
+                                                           X2);
+    ITG_LAUNCH_CHECK(ctx);
+    rc = itg_gemm_internal(ctx, ITG_F64, ITG_MODE_NATIVE, 'N', 'N', m2, k, nbp, -1.0, V1, n, X2,
+                           nbp, 1.0, U + (p + 1), n);
+    if (rc) return rc;
+  }
+  return ITG_OK;
+}
+
 extern "C" int itg_eigh_trunc(itg_ctx* ctx, int dtype, int64_t n64, void* Ain, double* w,
                               void* Uout, int64_t maxdim, double cutoff, int abs_cut, int rel_cut,
                               int64_t* kout, double* truncerr, double* docut) {
@@ -864,23 +1028,27 @@ extern "C" int itg_eigh_trunc(itg_ctx* ctx, int dtype, int64_t n64, void* Ain, d
   int n = (int)n64;
   size_t nn = (size_t)n * n;
   bool f32 = dtype == ITG_F32;
+  const size_t Tw = (size_t)(n + 63) / 64;   // tile rows of the tiled symv
   size_t bytes = (3 * nn + (f32 ? 2 * nn : 0)) * sizeof(double) + 3 * (size_t)n * NB * 8 +
+                 (size_t)((n + NB - 1) / NB) * NB * NB * 8 +
                  (size_t)n * 8 * 26 + 4 * (size_t)n * NB * 8 + 4 * NB * NB * 8 + 64 * 1024 +
-                 64 * 256;
+                 (Tw * n + Tw * (Tw + 1) / 2 + 64) * 8 + 64 * 256;
   void* ws;
   rc = eig_workspace(ctx, bytes, &ws);
   if (rc) return rc;
   Carver cv{(char*)ws};
   double* Qa = cv.take<double>(nn);
   double* Qb = cv.take<double>(nn);
-  double* Vbuf = cv.take<double>(nn);
+  // D&C merge vectors; afterwards the back-transform's compact-WY factors
+  double* Vbuf = cv.take<double>(std::max(nn, (size_t)((n + NB - 1) / NB) * NB * NB));
   double* A = f32 ? cv.take<double>(nn) : (double*)Ain;
   double* U = f32 ? cv.take<double>(nn) : (double*)Uout;
+  const size_t Tmax = (size_t)(n + TS - 1) / TS;
   double* VW = cv.take<double>(3 * (size_t)n * NB);
-  double* y = cv.take<double>(n);
+  double* Ypart = cv.take<double>(Tmax * n);
   double* ab = cv.take<double>(2 * NB);
   double* pnorm = cv.take<double>((size_t)n / TRD_ROWS + 1);
-  double* pyv = cv.take<double>((size_t)n / 4 + 2 * NB);
+  double* pyv = cv.take<double>(Tmax * (Tmax + 1) / 2 + 64);
   double* d = cv.take<double>(n);
   double* e = cv.take<double>(n);
   double* tau = cv.take<double>(n);
@@ -905,7 +1073,7 @@ extern "C" int itg_eigh_trunc(itg_ctx* ctx, int dtype, int64_t n64, void* Ain, d
     if (rc) return rc;
   }
   // 1. tridiagonalise
-  rc = tridiagonalize(ctx, n, A, d, e, tau, VW, y, ab, pnorm, pyv);
+  rc = tridiagonalize(ctx, n, A, d, e, tau, VW, Ypart, ab, pnorm, pyv);
   if (rc) return rc;
   // 2. divide and conquer
   double *Q, *D;
@@ -934,7 +1102,7 @@ extern "C" int itg_eigh_trunc(itg_ctx* ctx, int dtype, int64_t n64, void* Ain, d
   // 4. kept eigenvectors: gather, back-transform
   gather_cols_kernel<<<itg_blocks((int64_t)n * k, 256, 16384), 256, 0, ctx->stream>>>(n, (int)k, Q, perm, U);
   ITG_LAUNCH_CHECK(ctx);
-  rc = back_transform(ctx, n, n - 1, 1, (int)k, A, tau, U, Vbuf, Gm, Tm, X, X2);
+  rc = back_transform_trd(ctx, n, (int)k, A, tau, U, Vbuf, X, X2);
   if (rc) return rc;
   if (f32) {
     rc = itg_convert(ctx, (int64_t)n * k, ITG_F64, U, ITG_F32, Uout);
