@@ -867,16 +867,19 @@ static int tridiagonalize(itg_ctx* ctx, int n, double* A, double* d, double* e, 
       int m2 = n - q;
       double* A22 = A + (size_t)q * n + q;
       int rc;
+      // the tiled symv reads A22's 64 x 64 tiles on and below its diagonal
+      // (diagonal tiles whole), the post kernel and the column norms the
+      // lower part: the lower triangle plus one 128-tile band suffices
       if (nbp == NB) {
         rc = itg_gemm_internal(ctx, ITG_F64, ITG_MODE_NATIVE, 'N', 'T', m2, m2, 2 * NB, -1.0,
-                               Vp + q, n, Wp + q, n, 1.0, A22, n);
+                               Vp + q, n, Wp + q, n, 1.0, A22, n, true);
         if (rc) return rc;
       } else {
         rc = itg_gemm_internal(ctx, ITG_F64, ITG_MODE_NATIVE, 'N', 'T', m2, m2, nbp, -1.0, Vp + q,
-                               n, Wp + q, n, 1.0, A22, n);
+                               n, Wp + q, n, 1.0, A22, n, true);
         if (rc) return rc;
         rc = itg_gemm_internal(ctx, ITG_F64, ITG_MODE_NATIVE, 'N', 'T', m2, m2, nbp, -1.0, Wp + q,
-                               n, Vp + q, n, 1.0, A22, n);
+                               n, Vp + q, n, 1.0, A22, n, true);
         if (rc) return rc;
       }
     }

@@ -24,6 +24,9 @@ struct GettArgs {
   int64_t kchunk;   // K range per blockIdx.z (set by the launcher)
   void* partial;    // split-K partials [z][n][m] in the compute type, or null
   int64_t tilesM, tilesN;
+  // 1: skip output tiles (tm, tn) with tm > tn + 1 (callers that need only
+  // the lower triangle plus one tile band of a square C; no split-K)
+  int band;
 };
 
 // Launch C(m,n) = alpha * sum_k A(m,k) B(k,n) + beta * C(m,n) with the

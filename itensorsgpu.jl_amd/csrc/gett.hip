@@ -97,6 +97,7 @@ __global__ __launch_bounds__(256, 2) void gett_kernel(GettArgs g) {
 
   int64_t tm, tn;
   tile_coords(g.tilesM, g.tilesN, tm, tn);
+  if (g.band && tm > tn + 1) return;   // whole workgroup, before any barrier
   const int64_t m0 = tm * BM, n0 = tn * BN;
   const int64_t kbeg = (int64_t)blockIdx.z * g.kchunk;
   const int64_t kend = (kbeg + g.kchunk < g.K) ? kbeg + g.kchunk : g.K;
@@ -540,7 +541,7 @@ static int launch_gett_t(itg_ctx* ctx, GettArgs& g, bool ak, bool bk) {
   // split K when the output has too few tiles to fill the chip
   int nsplit = 1;
   int64_t target = 2 * (int64_t)std::max(ctx->num_cu, 1);
-  if (tiles < target && g.K >= TR::BKT * 16) {
+  if (!g.band && tiles < target && g.K >= TR::BKT * 16) {
     // at least 4 k-tiles per split
     int64_t want = (target + tiles - 1) / tiles;
     int64_t maxs = g.K / (TR::BKT * 4);
@@ -683,7 +684,7 @@ extern "C" int itg_contract(itg_ctx* ctx, int dtype, int mode, int rA, const int
 
 int itg_gemm_internal(itg_ctx* ctx, int dtype, int mode, char ta, char tb, int64_t m, int64_t n,
                       int64_t k, double alpha, const void* A, int64_t lda, const void* B,
-                      int64_t ldb, double beta, void* C, int64_t ldc) {
+                      int64_t ldb, double beta, void* C, int64_t ldc, bool lower_band) {
   if (m == 0 || n == 0) return ITG_OK;
   if (k == 0 || alpha == 0.0) {
     // C = beta*C on an m x n window with leading dimension ldc
@@ -706,6 +707,9 @@ int itg_gemm_internal(itg_ctx* ctx, int dtype, int mode, char ta, char tb, int64
   g.sCm = ldc; g.sCn = 1;
   g.alpha = alpha;
   g.beta = beta;
+  // the kernel's (tm, tn) is the caller's (column block, row block): the
+  // caller's rows >= columns - 1 tile band is tm <= tn + 1
+  g.band = (lower_band && m == n) ? 1 : 0;
   bool ak = (g.sAm != 1) && (g.sAk == 1);
   bool bk = (g.sBn != 1) && (g.sBk == 1);
   return launch_gett(ctx, dtype, mode, g, ak, bk);

@@ -97,10 +97,13 @@ static inline unsigned itg_blocks(int64_t n, int threads, int64_t cap = 65535 * 
 }
 
 // Internal (non-exported) entry points used across translation units.
+// lower_band: compute only the 128 x 128 output tiles on or below the
+// diagonal and the first tile band above it (square C whose upper part far
+// from the diagonal is never read)
 int itg_gemm_internal(itg_ctx* ctx, int dtype, int mode, char ta, char tb,
                       int64_t m, int64_t n, int64_t k, double alpha,
                       const void* A, int64_t lda, const void* B, int64_t ldb,
-                      double beta, void* C, int64_t ldc);
+                      double beta, void* C, int64_t ldc, bool lower_band = false);
 void itg_plan_cache_clear(itg_ctx* ctx);
 // Event bracket of one launch for itg_timing: t = itg_tstart(ctx, kind)
 // before the launch, itg_tstop(ctx, t, kind, flops, bytes) after it (no-ops
