@@ -91,7 +91,8 @@ def run_mode(mode, N, chi, steps, warmup, krylov, config3=False, timed_kinds=())
     kernels = {_lib.KERNEL_NAMES[k]: _lib.ctx().timing_report(k) for k in timed_kinds}
     _lib.ctx().timing(())
     return {"s_per_sweep": (t2 - t1) / steps, "energy": E, "maxlinkdim": psi.maxlinkdim(),
-            "setup_s": setup_s, "phases": timer.report(), "kernels": kernels, "sweeps": rec}
+            "setup_s": setup_s, "phases": timer.report(), "sweeps": rec,
+            "kernels": {k: v for k, v in kernels.items() if v["launches"]}}
 
 
 def eigensolver_kernels(n):
@@ -133,6 +134,35 @@ def precision_check(N, chi, nsweep, krylov, modes):
         log("[%s] precision check: E = %.15f after %d sweeps (%.1f s)"
             % (m, out[m], nsweep, time.perf_counter() - t0))
     return out
+
+
+def kernel_sample(chi, reps=3):
+    """Short fixed workload for hardware-counter passes: an in-place scale of
+    2^27 doubles (exactly 1 GiB read and 1 GiB written: the calibration of
+    FETCH_SIZE/WRITE_SIZE), then `reps` prepared split-BF16 Heff applies at
+    the bulk-bond shape (chi, w=5, d=2). Counter passes serialise every
+    dispatch, so the sample stays at a few dozen launches."""
+    import torch
+    from itg import ops
+    x = ops.randn(1 << 27, 1)
+    for _ in range(3):
+        ops.scal(x, 0.5)
+    torch.cuda.synchronize()
+    del x
+    w, d = 5, 2
+    L = ops.randn(chi * w * chi, 11)
+    R = ops.randn(chi * w * chi, 12)
+    W1 = ops.randn(w * w * d * d, 13)
+    W2 = ops.randn(w * w * d * d, 14)
+    psi = ops.randn(chi * d * d * chi, 15)
+    y = ops.empty(chi * d * d * chi)
+    work = ops.heff2_work(chi, w, w, w, d, chi)
+    dims = (chi, w, w, w, d, chi)
+    prep = ops.heff2_prepare(L, W1, W2, R, *dims, mode="bf16x3")
+    for _ in range(reps):
+        ops.heff2_apply_prepared(L, R, prep, psi, *dims, mode="bf16x3", out=y, work=work)
+    torch.cuda.synchronize()
+    log("kernel sample done")
 
 
 def cut_dims(N, d, chi):
@@ -227,6 +257,8 @@ def main():
     ap.add_argument("--krylov", type=int, default=3)
     ap.add_argument("--modes", default="bf16x3,native")
     ap.add_argument("--micro", action="store_true")
+    ap.add_argument("--kernel-sample", action="store_true",
+                    help="short fixed workload for rocprofv3 --pmc passes (see kernel_sample)")
     ap.add_argument("--config3", action="store_true",
                     help="config-3 start: link dims 10, ramp 64..1024, cutoff 1e-12")
     ap.add_argument("--cpu-chi", type=int, default=1024)
@@ -246,29 +278,49 @@ def main():
     if args.micro:
         print(json.dumps({"micro": {c: micro(chi=c) for c in (1024, 2048)}, "device": info}))
         return
+    if args.kernel_sample:
+        kernel_sample(args.chi)
+        return
     modes = args.modes.split(",")
     from itg import _lib as L_
     res = {}
     head = modes[0]
+    # the headline form's timed sweeps event-time every split-GEMM and W-pass
+    # launch (about 1.2k per sweep); they are the first such launches of the
+    # process, so a kernel trace of this command can be cut to the same set
+    timed = (L_.K_GEMM_SPLIT, L_.K_WPASS) if head == "bf16x3" else (L_.K_GETT, L_.K_WPASS)
     for m in modes:
-        kinds = ((L_.K_GEMM_SPLIT, L_.K_WPASS) if m == "bf16x3" else ()) if m == head else ()
         res[m] = run_mode(m, args.N, args.chi, args.steps, args.warmup, args.krylov,
-                          args.config3, timed_kinds=kinds)
+                          args.config3, timed_kinds=timed if m == head else ())
         log(m, {k: v for k, v in res[m].items() if k != "sweeps"})
     r = res[head]
     heff = r["phases"].get("heff", {"ms": float("nan"), "flops": 0.0, "calls": 0})
+    pc = None
+    if len(modes) > 1 and args.pc_chi > 0:
+        pc = precision_check(args.N, args.pc_chi, args.pc_sweeps, args.krylov, modes)
+    kern = r.get("kernels", {})
+    mfma_kern = {k: v for k, v in kern.items()
+                 if k != L_.KERNEL_NAMES[L_.K_WPASS] and v["launches"]}
     eig = eigensolver_kernels(2 * args.chi)
     log("eigensolver", eig)
-    # dominant event-timed kernel of the headline form
-    kern = r.get("kernels") or {}
-    if kern:
-        kname, kd = max(kern.items(), key=lambda kv: kv[1]["ms"])
+    if mfma_kern:
+        kname, kd = max(mfma_kern.items(), key=lambda kv: kv[1]["ms"])
         kms = kd["ms"] / max(kd["launches"], 1)
         roof = {"bound": "mfma", "kernel": kname, "achieved": kd["flops"] / kd["ms"] / 1e9,
                 "peak": PEAK[head], "unit": "TFLOP/s", "traffic": None,
                 "launches": kd["launches"], "avg_ms": kms,
-                "flops_per_launch": kd["flops"] / max(kd["launches"], 1)}
+                "flops_per_launch": kd["flops"] / max(kd["launches"], 1),
+                "launch_set": "every launch of the timed sweeps (the process's first %d)"
+                              % kd["launches"]}
         roof["frac"] = roof["achieved"] / roof["peak"]
+        # HBM bytes per launch from the committed FETCH_SIZE/WRITE_SIZE passes
+        # (bulk-bond shapes; see the file for the commands and calibration)
+        tpath = os.path.join(ROOT, "profiles", "gemm_split_kernel_traffic_chi2048.json")
+        if kname.startswith("gemm_split_kernel") and os.path.exists(tpath):
+            tr = json.load(open(tpath))
+            roof["traffic"] = tr["bytes_per_launch"]
+            roof["traffic_shape"] = tr["shape"]
+            roof["traffic_algorithmic"] = tr["algorithmic_bytes_per_launch"]
     else:
         achieved = heff["flops"] / (heff["ms"] / 1e3) / 1e12 if heff["ms"] > 0 else float("nan")
         roof = {"bound": "mfma", "kernel": "two-site Heff apply (C1 + W pass + C4)",
@@ -308,6 +360,7 @@ def main():
                        "tflops": heff["flops"] / max(heff["ms"], 1e-9) / 1e9},
         "eigensolver": eig,
         "results": {m: {k: v for k, v in res[m].items() if k != "sweeps"} for m in modes},
+        "kernels": kern,
     }
     if len(modes) > 1:
         e = [res[m]["energy"] for m in modes]
@@ -315,8 +368,7 @@ def main():
         # after one sweep from a random state the two forms follow different
         # Davidson trajectories, so this difference is not a precision figure
         line["dE_per_site_timed_sweep"] = abs(e[0] - e[1]) / args.N
-        if args.pc_chi > 0:
-            pc = precision_check(args.N, args.pc_chi, args.pc_sweeps, args.krylov, modes)
+        if pc is not None:
             line["dE_per_site"] = abs(pc[modes[0]] - pc[modes[1]]) / args.N
             line["precision_check"] = {
                 "workload": "same chain, random MPS link dims 10 seed 2, %d sweeps, maxdim ramp "

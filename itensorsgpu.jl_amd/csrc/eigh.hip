@@ -498,6 +498,45 @@ __global__ __launch_bounds__(256) void bt_apply_t_kernel(int nb, int k, const do
   X2[(size_t)c * nb + r] = s;
 }
 
+// Back-transform of a group of BTG consecutive panels (up to BTG*NB
+// reflectors) at once. With Y = V_G^T U (the group's rows of U before the
+// group is applied) and the Gram matrix Gg = V_G^T V_G, the panels applied
+// last-to-first give X_b = Y_b - sum_{b' > b} Gg(b, b') X2_b' and
+// X2_b = T_b X_b; then U -= V_G X2 in one product. One block: 8 columns of
+// U x the 32 rows of the current panel; X2 of the later panels in LDS.
+constexpr int BTG = 4;
+
+__global__ __launch_bounds__(256) void bt_group_solve_kernel(int nbg, int k,
+                                                             const double* __restrict__ Y,
+                                                             const double* __restrict__ Gg,
+                                                             const double* __restrict__ Tall,
+                                                             double* __restrict__ X2) {
+  __shared__ double X2s[8][BTG * NB];
+  __shared__ double xs[8][NB];
+  const int t = threadIdx.x, r = t % NB, cl = t / NB;
+  const int c = blockIdx.x * 8 + cl;
+  const bool live = c < k;
+  const int npb = (nbg + NB - 1) / NB;
+  for (int b = npb - 1; b >= 0; --b) {
+    const int r0 = b * NB, nb = min(NB, nbg - r0);
+    double x = 0.0;
+    if (live && r < nb) {
+      x = Y[(size_t)c * nbg + r0 + r];
+      for (int q = r0 + nb; q < nbg; ++q) x -= Gg[(r0 + r) + (size_t)q * nbg] * X2s[cl][q];
+    }
+    xs[cl][r] = x;
+    __syncthreads();
+    if (live && r < nb) {
+      const double* T = Tall + (size_t)b * NB * NB;   // group-relative panel b
+      double v = 0.0;
+      for (int q = r; q < nb; ++q) v += T[r + q * nb] * xs[cl][q];
+      X2s[cl][r0 + r] = v;
+      X2[(size_t)c * nbg + r0 + r] = v;
+    }
+    __syncthreads();
+  }
+}
+
 // --------------------------------------------------------- divide & conquer
 // Q = I and Qn = 0: every merge rewrites whole block squares, and entries
 // outside the current squares of both ping-pong buffers must stay zero
@@ -984,29 +1023,34 @@ static int back_transform(itg_ctx* ctx, int n, int nrefl, int off, int k, const 
 
 // U (n x k) <- Q U for the tridiagonalisation's Q = H_0 ... H_{n-2}, with
 // the reflectors read in place from A (after bt_clear_upper) and every T
-// formed up front by bt_tall_kernel; per panel X = V^T U, X2 = T X,
-// U -= V X2.
+// formed up front by bt_tall_kernel. Groups of BTG panels go last-to-first:
+// Y = V_G^T U, Gg = V_G^T V_G, the group solve for X2, U -= V_G X2.
 static int back_transform_trd(itg_ctx* ctx, int n, int k, double* A, const double* tau,
-                              double* U, double* Tall, double* X, double* X2) {
+                              double* U, double* Tall, double* Y, double* X2, double* Gg) {
   int nrefl = n - 1;
   if (nrefl < 1 || k < 1) return ITG_OK;
   bt_clear_upper_kernel<<<itg_blocks((int64_t)n * n, 256, 16384), 256, 0, ctx->stream>>>(A, n);
   int npan = (nrefl + NB - 1) / NB;
   bt_tall_kernel<<<npan, 256, 0, ctx->stream>>>(A, n, nrefl, tau, Tall);
   ITG_LAUNCH_CHECK(ctx);
-  for (int b = npan - 1; b >= 0; --b) {
-    int p = b * NB;
-    int nbp = std::min(NB, nrefl - p);
-    int m2 = n - p - 1;
-    const double* V1 = A + (size_t)p * n + (p + 1);   // rows p+1.., columns p..p+nbp
-    int rc = itg_gemm_internal(ctx, ITG_F64, ITG_MODE_NATIVE, 'T', 'N', nbp, k, m2, 1.0, V1, n,
-                               U + (p + 1), n, 0.0, X, nbp);
+  const int gsz = BTG * NB;
+  int ngrp = (nrefl + gsz - 1) / gsz;
+  for (int g = ngrp - 1; g >= 0; --g) {
+    int p0 = g * gsz;
+    int nbg = std::min(gsz, nrefl - p0);
+    int m2 = n - p0 - 1;
+    const double* VG = A + (size_t)p0 * n + (p0 + 1);   // rows p0+1.., columns p0..p0+nbg
+    int rc = itg_gemm_internal(ctx, ITG_F64, ITG_MODE_NATIVE, 'T', 'N', nbg, k, m2, 1.0, VG, n,
+                               U + (p0 + 1), n, 0.0, Y, nbg);
     if (rc) return rc;
-    bt_apply_t_kernel<<<(k + 7) / 8, 256, 0, ctx->stream>>>(nbp, k, Tall + (size_t)b * NB * NB, X,
-                                                           X2);
+    rc = itg_gemm_internal(ctx, ITG_F64, ITG_MODE_NATIVE, 'T', 'N', nbg, nbg, m2, 1.0, VG, n, VG,
+                           n, 0.0, Gg, nbg);
+    if (rc) return rc;
+    bt_group_solve_kernel<<<(k + 7) / 8, 256, 0, ctx->stream>>>(
+        nbg, k, Y, Gg, Tall + (size_t)(p0 / NB) * NB * NB, X2);
     ITG_LAUNCH_CHECK(ctx);
-    rc = itg_gemm_internal(ctx, ITG_F64, ITG_MODE_NATIVE, 'N', 'N', m2, k, nbp, -1.0, V1, n, X2,
-                           nbp, 1.0, U + (p + 1), n);
+    rc = itg_gemm_internal(ctx, ITG_F64, ITG_MODE_NATIVE, 'N', 'N', m2, k, nbg, -1.0, VG, n, X2,
+                           nbg, 1.0, U + (p0 + 1), n);
     if (rc) return rc;
   }
   return ITG_OK;
@@ -1033,6 +1077,7 @@ extern "C" int itg_eigh_trunc(itg_ctx* ctx, int dtype, int64_t n64, void* Ain, d
   bool f32 = dtype == ITG_F32;
   const size_t Tw = (size_t)(n + 63) / 64;   // tile rows of the tiled symv
   size_t bytes = (3 * nn + (f32 ? 2 * nn : 0)) * sizeof(double) + 3 * (size_t)n * NB * 8 +
+                 (2 * (size_t)BTG * NB * n + (size_t)BTG * NB * BTG * NB) * 8 +
                  (size_t)((n + NB - 1) / NB) * NB * NB * 8 +
                  (size_t)n * 8 * 26 + 4 * (size_t)n * NB * 8 + 4 * NB * NB * 8 + 64 * 1024 +
                  (Tw * n + Tw * (Tw + 1) / 2 + 64) * 8 + 64 * 256;
@@ -1067,8 +1112,9 @@ extern "C" int itg_eigh_trunc(itg_ctx* ctx, int dtype, int64_t n64, void* Ain, d
   int64_t* tab = cv.take<int64_t>(n);
   double* Gm = cv.take<double>(NB * NB);
   double* Tm = cv.take<double>(NB * NB);
-  double* X = cv.take<double>((size_t)NB * n);
-  double* X2 = cv.take<double>((size_t)NB * n);
+  double* Ygrp = cv.take<double>((size_t)BTG * NB * n);
+  double* X2grp = cv.take<double>((size_t)BTG * NB * n);
+  double* Ggrp = cv.take<double>((size_t)BTG * NB * BTG * NB);
   double* out3 = cv.take<double>(4);
   if (cv.off > bytes) return itg_fail(ctx, ITG_EOOM, "eigh: workspace layout overflow");
   if (f32) {
@@ -1105,7 +1151,7 @@ extern "C" int itg_eigh_trunc(itg_ctx* ctx, int dtype, int64_t n64, void* Ain, d
   // 4. kept eigenvectors: gather, back-transform
   gather_cols_kernel<<<itg_blocks((int64_t)n * k, 256, 16384), 256, 0, ctx->stream>>>(n, (int)k, Q, perm, U);
   ITG_LAUNCH_CHECK(ctx);
-  rc = back_transform_trd(ctx, n, (int)k, A, tau, U, Vbuf, X, X2);
+  rc = back_transform_trd(ctx, n, (int)k, A, tau, U, Vbuf, Ygrp, X2grp, Ggrp);
   if (rc) return rc;
   if (f32) {
     rc = itg_convert(ctx, (int64_t)n * k, ITG_F64, U, ITG_F32, Uout);
