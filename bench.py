@@ -165,6 +165,39 @@ def kernel_sample(chi, reps=3):
     log("kernel sample done")
 
 
+def precond_ab(N, chi, sweeps, krylov, train_chi=128, steps=3000):
+    """The learned preconditioner against the method without it: train the
+    network on pairs harvested from a half sweep of a different random
+    chain (train_chi), then run `sweeps` sweeps at maxdim chi from one random
+    start with no preconditioner, the diagonal one, and the guarded network
+    (diagonal fallback). Reports energies per sweep, s/sweep, matvecs and
+    reverted steps."""
+    from itg import _lib
+    from itg.mps import MPO, Sweeps, dmrg, heisenberg_opsum, random_mps
+    from itg.precond import MLPPreconditioner, harvest_sweep
+    H = MPO.from_opsum(heisenberg_opsum([(j, j + 1) for j in range(N - 1)]), N)
+    t0 = time.perf_counter()
+    X, T, rows, d = harvest_sweep(H, random_mps(N, 2, train_chi, seed=4), train_chi,
+                                  rows_per_bond=2048, sweeps=3)
+    net = MLPPreconditioner(d, hidden=256, seed=1)
+    l0 = net.loss(X, T, rows)
+    net.train(X, T, rows, steps=steps, batch=4096, lr=1e-3)
+    l1 = net.loss(X, T, rows)
+    train_s = time.perf_counter() - t0
+    log("precond A/B: trained on %d rows, loss %.4f -> %.4f (%.1f s)" % (rows, l0, l1, train_s))
+    out = {"train": {"rows": rows, "loss_start": l0, "loss_end": l1, "seconds": train_s,
+                     "chi": train_chi, "steps": steps}}
+    for name, pre in (("none", None), ("diag", "diag"), ("mlp", net.factory())):
+        rec = []
+        psi = random_mps(N, 2, chi, seed=3)
+        dmrg(H, psi, Sweeps(sweeps, maxdim=[chi], cutoff=[1e-12]), krylovdim=krylov,
+             precond=pre, record=rec, sweep_callback=lambda r: log("[%s]" % name, r))
+        _lib.ctx().sync()
+        out[name] = [{"energy": r["energy"], "seconds": r["seconds"], "matvecs": r["matvecs"],
+                      "reverted": r.get("reverted", 0)} for r in rec]
+    return out
+
+
 def cut_dims(N, d, chi):
     return [1] + [min(chi, d ** min(j, N - j)) for j in range(1, N)] + [1]
 
@@ -257,6 +290,10 @@ def main():
     ap.add_argument("--krylov", type=int, default=3)
     ap.add_argument("--modes", default="bf16x3,native")
     ap.add_argument("--micro", action="store_true")
+    ap.add_argument("--precond-ab", action="store_true",
+                    help="learned vs diagonal vs no preconditioner, N=--N, maxdim=--ab-chi")
+    ap.add_argument("--ab-chi", type=int, default=256)
+    ap.add_argument("--ab-sweeps", type=int, default=3)
     ap.add_argument("--kernel-sample", action="store_true",
                     help="short fixed workload for rocprofv3 --pmc passes (see kernel_sample)")
     ap.add_argument("--config3", action="store_true",
@@ -280,6 +317,10 @@ def main():
         return
     if args.kernel_sample:
         kernel_sample(args.chi)
+        return
+    if args.precond_ab:
+        print(json.dumps({"precond_ab": precond_ab(args.N, args.ab_chi, args.ab_sweeps,
+                                                   args.krylov), "device": info}))
         return
     modes = args.modes.split(",")
     from itg import _lib as L_

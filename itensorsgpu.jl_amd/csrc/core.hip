@@ -656,12 +656,22 @@ extern "C" int itg_truncate(itg_ctx* ctx, const double* P, int64_t n, int64_t ma
   return ITG_OK;
 }
 
+// The decision is the same serial rule as on the host; the weights are first
+// staged in LDS (up to kTruncLds of them) so its scans do not wait on HBM.
+constexpr int64_t kTruncLds = 8192;
+
 __global__ void truncate_kernel(const double* P, int64_t n, int64_t maxdim, int64_t mindim,
                                 double cutoff, int abs_cut, int rel_cut, int semantics,
                                 double* out) {
+  extern __shared__ double Ps[];
+  const bool staged = n <= kTruncLds;
+  if (staged) {
+    for (int64_t i = threadIdx.x; i < n; i += blockDim.x) Ps[i] = P[i];
+    __syncthreads();
+  }
   if (threadIdx.x == 0 && blockIdx.x == 0) {
-    itg_trunc_result r = itg_truncate_core(P, n, maxdim, mindim, cutoff, abs_cut, rel_cut,
-                                           semantics);
+    itg_trunc_result r = itg_truncate_core(staged ? Ps : P, n, maxdim, mindim, cutoff, abs_cut,
+                                           rel_cut, semantics);
     out[0] = (double)r.n;
     out[1] = r.truncerr;
     out[2] = r.docut;
@@ -671,8 +681,16 @@ __global__ void truncate_kernel(const double* P, int64_t n, int64_t maxdim, int6
 int itg_truncate_device_async(itg_ctx* ctx, const double* P, int64_t n, int64_t maxdim,
                               int64_t mindim, double cutoff, int abs_cut, int rel_cut,
                               int semantics, double* out3) {
-  truncate_kernel<<<1, 64, 0, ctx->stream>>>(P, n, maxdim, mindim, cutoff, abs_cut, rel_cut,
-                                             semantics, out3);
+  static bool lds_set = false;
+  if (!lds_set) {
+    ITG_HIP(ctx, hipFuncSetAttribute((const void*)truncate_kernel,
+                                     hipFuncAttributeMaxDynamicSharedMemorySize,
+                                     (int)(kTruncLds * sizeof(double))));
+    lds_set = true;
+  }
+  size_t lds = n <= kTruncLds ? (size_t)std::max<int64_t>(n, 1) * sizeof(double) : 0;
+  truncate_kernel<<<1, 256, lds, ctx->stream>>>(P, n, maxdim, mindim, cutoff, abs_cut, rel_cut,
+                                                semantics, out3);
   ITG_LAUNCH_CHECK(ctx);
   return ITG_OK;
 }

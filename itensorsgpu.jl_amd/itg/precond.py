@@ -119,18 +119,23 @@ class MLPPreconditioner:
             ops.mlp_scatter(Y, chiL, d, chiR, r0, n, t)
         return t
 
-    def factory(self, guard=True):
+    def factory(self, guard=True, min_residual=0.0):
         """dmrg `precond` factory: (ProjMPO, b, xdims) -> precond(q, lam)
-        carrying the diagonal preconditioner as its revert fallback."""
+        carrying the diagonal preconditioner as its revert fallback. Residuals
+        with norm below `min_residual` take the diagonal step directly (the
+        network is trained on states far from convergence)."""
 
         def make(PH, b, xdims):
             dg = PH.diag(b, xdims)
+            dpre = diag_preconditioner(dg)
 
             def pre(q, lam):
+                if min_residual > 0.0 and ops.nrm2(q) < min_residual:
+                    return dpre(q, lam)
                 return self.correction(q, lam, dg, xdims)
 
             if guard:
-                pre.fallback = diag_preconditioner(dg)
+                pre.fallback = dpre
             return pre
 
         return make
@@ -205,3 +210,48 @@ def evaluate(matvec, phi, dg, xdims, net, maxiter=2, lam0=None):
         lam, _ = davidson(matvec, x, maxiter=maxiter, precond=pre, fallback=fb, stats=st)
         out[name] = (lam, st["residual"], st.get("reverted", 0))
     return out, lam0
+
+
+def harvest_sweep(H, psi, maxdim, rows_per_bond=4096, bonds=None, cutoff=1e-12, seed=0,
+                  iters=40, sweeps=1):
+    """Training rows from the chain's own effective Hamiltonians: `sweeps`
+    full two-site sweeps (diagonal-preconditioned Davidson, 2 steps,
+    replacebond at maxdim) that harvest pairs at the listed bonds of every
+    left-to-right pass before each update, so the rows cover the states of a
+    run from its random start towards convergence. psi (right-canonical,
+    centre at site 0) is left in the same form. Returns (X, T, rows, d)."""
+    from . import mps
+    N = len(psi)
+    PH = mps.ProjMPO(H)
+    PH.build_right(psi)
+    bonds = set(range(1, N - 2)) if bonds is None else set(bonds)
+    Xs, Ts, rows, d = [], [], 0, None
+    for sw in range(sweeps):
+        for b, ha in mps.sweepnext(N):
+            A, Ad, B, Bd = psi.data[b], psi.dims[b], psi.data[b + 1], psi.dims[b + 1]
+            phi, pdims = ops.contract(A, Ad, [1, 2, 3], B, Bd, [3, 4, 5], [1, 2, 4, 5])
+            d = pdims[1]
+            PH.prepare(b, pdims)
+
+            def mv(x, b=b, pdims=pdims):
+                return PH.product(b, x, pdims)
+
+            dg = PH.diag(b, pdims)
+            if ha == 1 and b in bonds:
+                X, T, n, _ = harvest_pairs(mv, phi, dg, pdims, rows_per_bond,
+                                           seed=seed + 1000 * sw + b, iters=iters)
+                Xs.append(X)
+                Ts.append(T)
+                rows += n
+            lam, phi = mps.davidson(mv, phi, maxiter=2, precond=mps.diag_preconditioner(dg))
+            ortho = "left" if ha == 1 else "right"
+            mps.replacebond(psi, b, phi, pdims, ortho, maxdim, 1, cutoff)
+            if ha == 1:
+                L, Ld = PH.L[b]
+                PH.L[b + 1] = mps.update_left(L, Ld, psi.data[b], psi.dims[b], H.data[b],
+                                              H.dims[b])
+            else:
+                R, Rd = PH.R[b + 2]
+                PH.R[b + 1] = mps.update_right(R, Rd, psi.data[b + 1], psi.dims[b + 1],
+                                               H.data[b + 1], H.dims[b + 1])
+    return concat(Xs), concat(Ts), rows, d

@@ -172,10 +172,45 @@ def test_guard_keeps_residual_decreasing_with_garbage_weights(itg):
 
 
 def test_learned_preconditioner_gain(itg, capsys):
-    """Train on pairs harvested from one random chain (chi 48), evaluate on
-    held-out bonds of another (chi 64): energy error after two Davidson
-    steps with no, diagonal and learned (guarded) preconditioners."""
-    from itg import ops
+    """Heisenberg N=100: train on pairs harvested from three sweeps of one
+    random chi=128 chain, then run DMRG at maxdim 256 from another random
+    start with the diagonal preconditioner and with the guarded network
+    (diagonal fallback): the learned form is closer to the ground state after
+    the first sweep, and both converge to the same energy."""
+    from itg.mps import MPO, Sweeps, dmrg, heisenberg_opsum, random_mps
+    from itg.precond import MLPPreconditioner, harvest_sweep
+    N, chi = 100, 256
+    H = MPO.from_opsum(heisenberg_opsum([(j, j + 1) for j in range(N - 1)]), N)
+    X, T, rows, d = harvest_sweep(H, random_mps(N, 2, 128, seed=21), 128, rows_per_bond=2048,
+                                  sweeps=3)
+    net = MLPPreconditioner(d, hidden=256, seed=1)
+    l0 = net.loss(X, T, rows)
+    net.train(X, T, rows, steps=3000, batch=4096, lr=1e-3)
+    l1 = net.loss(X, T, rows)
+    res = {}
+    for name, pre in (("diag", "diag"), ("mlp", net.factory())):
+        rec = []
+        dmrg(H, random_mps(N, 2, chi, seed=22), Sweeps(3, maxdim=[chi], cutoff=[1e-12]),
+             precond=pre, record=rec)
+        res[name] = rec
+    e0 = min(r["energy"] for k in res for r in res[k])
+    with capsys.disabled():
+        print("\nlearned preconditioner: loss %.4f -> %.4f on %d rows" % (l0, l1, rows))
+        for k in res:
+            print("  %-4s" % k, ["%.10f (rev %d)" % (r["energy"], r.get("reverted", 0))
+                                 for r in res[k]])
+    assert l1 < 0.8 * l0
+    # the gain that holds across trainings and starts: closer to the ground
+    # state after the first sweep from a random state (on two other seed
+    # pairs also after the second; see the preconditioner A/B of bench.py)
+    assert res["mlp"][0]["energy"] < res["diag"][0]["energy"]
+    # and, with the guard, the same converged energy to 1e-6 per site
+    assert abs(res["mlp"][-1]["energy"] - res["diag"][-1]["energy"]) < 1e-6 * N
+
+
+def test_learned_step_against_diag_per_bond(itg, capsys):
+    """Single Davidson runs (2 steps) on held-out bond operators: energy
+    error with no, diagonal and learned (guarded) preconditioners."""
     from itg.precond import MLPPreconditioner, concat, evaluate, harvest_pairs
     train_ops = _chain_operators(itg, 24, 48, 11, set(range(3, 20)))
     Xs, Ts, rows = [], [], 0
@@ -186,26 +221,14 @@ def test_learned_preconditioner_gain(itg, capsys):
         rows += n
     X, T = concat(Xs), concat(Ts)
     net = MLPPreconditioner(2, hidden=256, seed=1)
-    l0 = net.loss(X, T, rows)
     net.train(X, T, rows, steps=1500, batch=4096, lr=1e-3)
-    l1 = net.loss(X, T, rows)
-    test_ops = _chain_operators(itg, 24, 64, 12, {6, 9, 12, 15})
-    gains, rev = [], 0
     lines = []
-    for mv, phi, dg, pdims in test_ops:
+    for mv, phi, dg, pdims in _chain_operators(itg, 24, 64, 12, {6, 9, 12, 15}):
         res, lam0 = evaluate(mv, phi, dg, pdims, net, maxiter=2)
         e = {k: v[0] - lam0 for k, v in res.items()}
-        rev += res["mlp"][2]
-        gains.append(e["diag"] / max(e["mlp"], 1e-300))
+        assert all(np.isfinite(v) and v > -1e-9 for v in e.values())
         lines.append("%s plain %.3e diag %.3e mlp %.3e (reverted %d)" % (
             pdims, e["plain"], e["diag"], e["mlp"], res["mlp"][2]))
     with capsys.disabled():
-        print("\nlearned preconditioner: train loss %.4f -> %.4f on %d rows" % (l0, l1, rows))
         for ln in lines:
             print("  dE after 2 steps:", ln)
-        print("  diag/mlp energy-error ratio per bond:", ["%.2f" % g for g in gains])
-    assert l1 < l0
-    # the guarded learned step is never worse than the start
-    for mv, phi, dg, pdims in test_ops[:1]:
-        pass
-    assert np.all(np.isfinite(gains))
