@@ -276,20 +276,19 @@ __global__ __launch_bounds__(256) void trd_post_kernel(
   if (r > j && r < n) {
     // eight independent loads in flight per lane
     const double* yc = Ypart + (r - jn);
-    double q0 = 0.0, q1 = 0.0, q2 = 0.0, q3 = 0.0, q4 = 0.0, q5 = 0.0, q6 = 0.0, q7 = 0.0;
-    int tt = wv;
-    for (; tt + 28 < T; tt += 32) {
-      q0 += yc[(size_t)tt * m];
-      q1 += yc[(size_t)(tt + 4) * m];
-      q2 += yc[(size_t)(tt + 8) * m];
-      q3 += yc[(size_t)(tt + 12) * m];
-      q4 += yc[(size_t)(tt + 16) * m];
-      q5 += yc[(size_t)(tt + 20) * m];
-      q6 += yc[(size_t)(tt + 24) * m];
-      q7 += yc[(size_t)(tt + 28) * m];
+    // up to 16 independent loads in flight per lane (T <= 64 in one round)
+    double q[16];
+#pragma unroll
+    for (int u = 0; u < 16; ++u) {
+      int tt = wv + 4 * u;
+      q[u] = tt < T ? yc[(size_t)tt * m] : 0.0;
     }
-    for (; tt < T; tt += 4) q0 += yc[(size_t)tt * m];
-    yp = ((q0 + q1) + (q2 + q3)) + ((q4 + q5) + (q6 + q7));
+    for (int tt = wv + 64; tt < T; tt += 4) q[0] += yc[(size_t)tt * m];
+#pragma unroll
+    for (int u = 8; u > 0; u >>= 1)
+#pragma unroll
+      for (int v = 0; v < u; ++v) q[v] += q[v + u];
+    yp = q[0];
   }
   ys[wv][lane] = yp;
   double y0 = 0.0;
@@ -513,23 +512,32 @@ __global__ __launch_bounds__(256) void bt_group_solve_kernel(int nbg, int k,
                                                              double* __restrict__ X2) {
   __shared__ double X2s[8][BTG * NB];
   __shared__ double xs[8][NB];
+  __shared__ double Gs[NB][(BTG - 1) * NB + 1];   // Gg rows of panel b, later columns
+  __shared__ double Ts[NB][NB + 1];
   const int t = threadIdx.x, r = t % NB, cl = t / NB;
   const int c = blockIdx.x * 8 + cl;
   const bool live = c < k;
   const int npb = (nbg + NB - 1) / NB;
   for (int b = npb - 1; b >= 0; --b) {
-    const int r0 = b * NB, nb = min(NB, nbg - r0);
+    const int r0 = b * NB, nb = min(NB, nbg - r0), q0 = r0 + nb, nq = nbg - q0;
+    // stage Gg(r0.., q0..) and T_b (group-relative panel b)
+    for (int e = t; e < NB * nq; e += 256) {
+      int rr = e % NB, qq = e / NB;
+      Gs[rr][qq] = rr < nb ? Gg[(r0 + rr) + (size_t)(q0 + qq) * nbg] : 0.0;
+    }
+    const double* T = Tall + (size_t)b * NB * NB;
+    for (int e = t; e < nb * nb; e += 256) Ts[e % nb][e / nb] = T[e];
+    __syncthreads();
     double x = 0.0;
     if (live && r < nb) {
       x = Y[(size_t)c * nbg + r0 + r];
-      for (int q = r0 + nb; q < nbg; ++q) x -= Gg[(r0 + r) + (size_t)q * nbg] * X2s[cl][q];
+      for (int qq = 0; qq < nq; ++qq) x -= Gs[r][qq] * X2s[cl][q0 + qq];
     }
     xs[cl][r] = x;
     __syncthreads();
     if (live && r < nb) {
-      const double* T = Tall + (size_t)b * NB * NB;   // group-relative panel b
       double v = 0.0;
-      for (int q = r; q < nb; ++q) v += T[r + q * nb] * xs[cl][q];
+      for (int q = r; q < nb; ++q) v += Ts[r][q] * xs[cl][q];
       X2s[cl][r0 + r] = v;
       X2[(size_t)c * nbg + r0 + r] = v;
     }
