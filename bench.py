@@ -335,7 +335,8 @@ def main():
                           args.config3, timed_kinds=timed if m == head else ())
         log(m, {k: v for k, v in res[m].items() if k != "sweeps"})
     r = res[head]
-    heff = r["phases"].get("heff", {"ms": float("nan"), "flops": 0.0, "calls": 0})
+    heff = r["phases"].get("heff") or r["phases"].get("davidson") or \
+        {"ms": float("nan"), "flops": 0.0, "calls": 0}
     pc = None
     if len(modes) > 1 and args.pc_chi > 0:
         pc = precision_check(args.N, args.pc_chi, args.pc_sweeps, args.krylov, modes)
@@ -397,8 +398,8 @@ def main():
                    "N": args.N, "chi": args.chi, "mpo_bond_dim": 5, "modes": modes},
         "roofline": roof,
         "roofline_eigensolver": roof_eig,
-        "heff_apply": {"launches": heff["calls"], "avg_ms": heff["ms"] / max(heff["calls"], 1),
-                       "tflops": heff["flops"] / max(heff["ms"], 1e-9) / 1e9},
+        "davidson": {"bonds": heff["calls"], "avg_ms": heff["ms"] / max(heff["calls"], 1),
+                     "heff_tflops": heff["flops"] / max(heff["ms"], 1e-9) / 1e9},
         "eigensolver": eig,
         "results": {m: {k: v for k, v in res[m].items() if k != "sweeps"} for m in modes},
         "kernels": kern,

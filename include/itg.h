@@ -232,6 +232,22 @@ int itg_heff2_apply_prepared(itg_ctx* ctx, int dtype, int mode, int64_t chiL,
                              int64_t chiR, const void* L, const void* R,
                              const void* prep, const void* psi, void* out,
                              void* work);
+/* Davidson for the lowest eigenpair of the prepared operator (the eigsolve
+ * dmrg runs per bond, test/dmrg.jl:27 through ITensors' davidson), on the
+ * device: FP64 Krylov basis of up to maxiter+1 (<= 17) vectors, two
+ * Gram-Schmidt passes with batched dot products, the projected eigenproblem
+ * on the host, ITensors' stopping rule on the residual norm and errgoal.
+ * psi (FP64, chiL x d x d x chiR) is the start vector and receives the
+ * normalised Ritz vector; diag (the operator's diagonal) selects the
+ * diagonal preconditioner, null none. work: itg_heff2_work_elems(); kwork:
+ * itg_davidson2_work_elems() doubles. In ITG_MODE_BF16X3 the Heff apply is
+ * captured once per (shape, buffers) into a HIP graph and replayed. */
+int64_t itg_davidson2_work_elems(int64_t chiL, int64_t d, int64_t chiR, int maxiter);
+int itg_davidson2(itg_ctx* ctx, int dtype, int mode, int64_t chiL, int64_t wl,
+                  int64_t wm, int64_t wr, int64_t d, int64_t chiR, const void* L,
+                  const void* R, const void* prep, void* psi, int maxiter,
+                  double errgoal, const void* diag, void* work, void* kwork,
+                  double* energy, int* matvecs);
 /* Diagonal of the same operator, shape (chiL,d,d,chiR). */
 int itg_heff2_diag(itg_ctx* ctx, int dtype, int64_t chiL, int64_t wl,
                    int64_t wm, int64_t wr, int64_t d, int64_t chiR,

@@ -63,6 +63,7 @@ extern "C" int itg_destroy(itg_ctx* ctx) {
   hipSetDevice(ctx->device);
   hipStreamSynchronize(ctx->stream);
   itg_plan_cache_clear(ctx);
+  itg_graph_cache_clear(ctx);
   itg_timing_free(ctx);
   if (ctx->ws) hipFree(ctx->ws);
   if (ctx->eig_ws) hipFree(ctx->eig_ws);

@@ -282,6 +282,17 @@ extern "C" int itg_heff2_apply_prepared(itg_ctx* ctx, int dtype, int mode, int64
                carve_work(h, es, work));
 }
 
+int itg_heff2_apply_internal(itg_ctx* ctx, int dtype, int mode, const int64_t* dims,
+                             const void* L, const void* R, const void* prep, const void* psi,
+                             void* out, void* work) {
+  HeffDims h(dims[0], dims[1], dims[2], dims[3], dims[4], dims[5]);
+  int rc = check_heff(ctx, dtype, mode, h);
+  if (rc) return rc;
+  size_t es = itg_dtype_size(dtype);
+  return apply(ctx, dtype, mode, h, L, R, carve_prep(h, (void*)prep), psi, out,
+               carve_work(h, es, work));
+}
+
 extern "C" int itg_heff2_apply(itg_ctx* ctx, int dtype, int mode, int64_t chiL, int64_t wl,
                                int64_t wm, int64_t wr, int64_t d, int64_t chiR, const void* L,
                                const void* W1, const void* W2, const void* R, const void* psi,

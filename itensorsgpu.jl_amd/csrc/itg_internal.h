@@ -46,6 +46,8 @@ struct itg_ctx {
   std::vector<TimedLaunch> launches;
   std::vector<hipEvent_t> event_pool;
   size_t event_next = 0;
+  // captured Heff-apply graphs of itg_davidson2 (null: seen once, eager)
+  std::unordered_map<std::string, hipGraphExec_t> graphs;
 };
 
 constexpr int kRedSlots = 16384;
@@ -111,3 +113,9 @@ void itg_plan_cache_clear(itg_ctx* ctx);
 int itg_tstart(itg_ctx* ctx, int kind);
 void itg_tstop(itg_ctx* ctx, int t, int kind, double flops, double bytes);
 void itg_timing_free(itg_ctx* ctx);
+void itg_graph_cache_clear(itg_ctx* ctx);
+// Prepared two-site Heff apply (heff.hip) with dims = {chiL, wl, wm, wr, d,
+// chiR}; validates the shape, launches only.
+int itg_heff2_apply_internal(itg_ctx* ctx, int dtype, int mode, const int64_t* dims,
+                             const void* L, const void* R, const void* prep, const void* psi,
+                             void* out, void* work);

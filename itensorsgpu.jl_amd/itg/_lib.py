@@ -99,6 +99,10 @@ _SIGS = {
     "itg_heff2_prepare": (C.c_int, [_ctxp, C.c_int, C.c_int] + [C.c_int64] * 6 + [_vp] * 5),
     "itg_heff2_apply_prepared": (C.c_int, [_ctxp, C.c_int, C.c_int] + [C.c_int64] * 6 +
                                  [_vp] * 6),
+    "itg_davidson2_work_elems": (C.c_int64, [C.c_int64, C.c_int64, C.c_int64, C.c_int]),
+    "itg_davidson2": (C.c_int, [_ctxp, C.c_int, C.c_int] + [C.c_int64] * 6 +
+                      [_vp, _vp, _vp, _vp, C.c_int, C.c_double, _vp, _vp, _vp, _dp,
+                       C.POINTER(C.c_int)]),
     "itg_mlp_param_count": (C.c_int64, [C.c_int, C.c_int, C.c_int]),
     "itg_mlp_forward": (C.c_int, [_ctxp, C.c_int, C.c_int, C.c_int, C.c_int, _vp, _vp, _vp,
                                   _vp]),

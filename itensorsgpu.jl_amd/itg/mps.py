@@ -307,6 +307,7 @@ class ProjMPO:
         self.L = [None] * (N + 1)
         self.R = [None] * (N + 1)
         self._prep = None        # (bond, mode, dims, buffer) of itg_heff2_prepare
+        self._bufs = {}          # Heff and Krylov work buffers per shape (reused)
 
     def build_right(self, psi, upto=2):
         N = len(psi)
@@ -332,6 +333,32 @@ class ProjMPO:
         buf = ops.heff2_prepare(L, self.H.data[b], self.H.data[b + 1], R, *dims, mode=self.mode,
                                 prep=buf)
         self._prep = (b, self.mode, dims, buf, L, R)
+
+    def eigsolve(self, b, phi, xdims, maxiter=2, errgoal=1e-14, diag=None, stats=None):
+        """Native device Davidson (itg_davidson2) on the prepared bond b;
+        phi is overwritten with the Ritz vector. The work buffers are kept
+        per shape so that split-BF16 applies replay their captured graphs."""
+        L, Ld = self.L[b]
+        R, Rd = self.R[b + 2]
+        p = self._prep
+        if p is None or p[0] != b or p[1] != self.mode or p[4] is not L or p[5] is not R:
+            self.prepare(b, xdims)
+            p = self._prep
+        dims = p[2]
+        key = (dims, maxiter)
+        if key not in self._bufs:
+            self._bufs[key] = (ops.heff2_work(*dims, "float64"),
+                               ops.davidson2_work(dims[0], dims[4], dims[5], maxiter))
+        work, kwork = self._bufs[key]
+        ev = self.timer.start() if self.timer is not None else None
+        lam, nmv = ops.davidson2(L, R, p[3], phi, *dims, maxiter=maxiter, errgoal=errgoal,
+                                 diag=diag, mode=self.mode, work=work, kwork=kwork)
+        if ev is not None:
+            self.timer.stop("davidson", ev, nmv * heff_flops(dims[0], dims[1], dims[3], dims[4],
+                                                             dims[5]))
+        if stats is not None:
+            stats["matvecs"] = stats.get("matvecs", 0) + nmv
+        return lam, phi
 
     def product(self, b, x, xdims, out=None, work=None):
         L, Ld = self.L[b]
@@ -530,12 +557,14 @@ def sweepnext(N):
 
 
 def dmrg(H, psi, sweeps, mode="native", krylovdim=3, precond=None, errgoal=1e-14,
-         record=None, sweep_callback=None, env_mode=None, ph=None, timer=None):
+         record=None, sweep_callback=None, env_mode=None, ph=None, timer=None, native=True):
     """Two-site DMRG; psi (right-orthogonal, centre at site 0) is updated in
     place. Returns the final energy. `precond`: None, "diag" or a callable
     factory (PH, b, xdims) -> precond(q, lam). Passing the ProjMPO `ph` of a
     previous call continues from its environments (a completed sweep leaves
-    psi right-orthogonal and the right environments current)."""
+    psi right-orthogonal and the right environments current). With
+    `native` (default) the per-bond Davidson runs in libitg (itg_davidson2)
+    unless a learned preconditioner is given."""
     N = len(psi)
     # environments accumulate along the chain (L_b is b successive updates),
     # so their errors add up: they stay FP64 unless asked otherwise
@@ -559,22 +588,28 @@ def dmrg(H, psi, sweeps, mode="native", krylovdim=3, precond=None, errgoal=1e-14
             A, Ad = psi.data[b], psi.dims[b]
             B, Bd = psi.data[b + 1], psi.dims[b + 1]
             phi, pdims = ops.contract(A, Ad, [1, 2, 3], B, Bd, [3, 4, 5], [1, 2, 4, 5])
-            work = ops.heff2_work(pdims[0], H.dims[b][0], H.dims[b][1], H.dims[b + 1][1],
-                                  pdims[1], pdims[3], str(phi.dtype).split(".")[-1])
-
             PH.prepare(b, pdims)
+            if native and not callable(precond):
+                # device Davidson (itg_davidson2); phi is a fresh contraction
+                # output, overwritten with the Ritz vector
+                dg = PH.diag(b, pdims) if precond == "diag" else None
+                energy, phi = PH.eigsolve(b, phi, pdims, maxiter=krylovdim - 1,
+                                          errgoal=errgoal, diag=dg, stats=stats)
+            else:
+                work = ops.heff2_work(pdims[0], H.dims[b][0], H.dims[b][1], H.dims[b + 1][1],
+                                      pdims[1], pdims[3], str(phi.dtype).split(".")[-1])
 
-            def mv(x, b=b, pdims=pdims, work=work):
-                return PH.product(b, x, pdims, work=work)
+                def mv(x, b=b, pdims=pdims, work=work):
+                    return PH.product(b, x, pdims, work=work)
 
-            pre = None
-            if precond == "diag":
-                pre = diag_preconditioner(PH.diag(b, pdims))
-            elif callable(precond):
-                pre = precond(PH, b, pdims)
-            energy, phi = davidson(mv, phi, maxiter=krylovdim - 1, errgoal=errgoal,
-                                   precond=pre, stats=stats,
-                                   fallback=getattr(pre, "fallback", None))
+                pre = None
+                if precond == "diag":
+                    pre = diag_preconditioner(PH.diag(b, pdims))
+                elif callable(precond):
+                    pre = precond(PH, b, pdims)
+                energy, phi = davidson(mv, phi, maxiter=krylovdim - 1, errgoal=errgoal,
+                                       precond=pre, stats=stats,
+                                       fallback=getattr(pre, "fallback", None))
             ortho = "left" if ha == 1 else "right"
             ev = timer.start() if timer is not None else None
             terr, k = replacebond(psi, b, phi, pdims, ortho, sweeps.maxdim[sw],

@@ -284,6 +284,29 @@ def heff2_apply_prepared(L, R, prep, psi, chiL, wl, wm, wr, d, chiR, mode="nativ
     return out
 
 
+def davidson2_work(chiL, d, chiR, maxiter):
+    return empty(lib().itg_davidson2_work_elems(chiL, d, chiR, int(maxiter)), "float64")
+
+
+def davidson2(L, R, prep, psi, chiL, wl, wm, wr, d, chiR, maxiter=2, errgoal=1e-14, diag=None,
+              mode="native", work=None, kwork=None):
+    """Device Davidson on the prepared Heff; psi (FP64) is the start vector
+    and is overwritten with the normalised Ritz vector. Returns
+    (energy, matvecs)."""
+    if work is None:
+        work = heff2_work(chiL, wl, wm, wr, d, chiR, "float64")
+    if kwork is None:
+        kwork = davidson2_work(chiL, d, chiR, maxiter)
+    e = C.c_double()
+    nmv = C.c_int()
+    c = ctx()
+    c.check(lib().itg_davidson2(c.ptr, dtype_code(psi), _lib.MODES[mode], chiL, wl, wm, wr, d,
+                                chiR, ptr(L), ptr(R), ptr(prep), ptr(psi), int(maxiter),
+                                float(errgoal), ptr(diag) if diag is not None else None,
+                                ptr(work), ptr(kwork), C.byref(e), C.byref(nmv)))
+    return e.value, nmv.value
+
+
 def heff2_diag(L, W1, W2, R, chiL, wl, wm, wr, d, chiR):
     dt = str(L.dtype).split(".")[-1]
     out = empty(chiL * d * d * chiR, dt)
