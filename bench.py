@@ -335,7 +335,7 @@ def main():
                           args.config3, timed_kinds=timed if m == head else ())
         log(m, {k: v for k, v in res[m].items() if k != "sweeps"})
     r = res[head]
-    heff = r["phases"].get("heff") or r["phases"].get("davidson") or \
+    heff = r["phases"].get("davidson") or r["phases"].get("heff") or \
         {"ms": float("nan"), "flops": 0.0, "calls": 0}
     pc = None
     if len(modes) > 1 and args.pc_chi > 0:
