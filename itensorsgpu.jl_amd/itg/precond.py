@@ -9,9 +9,20 @@ delta = diag(Heff) - lambda (itg_mlp_features). It predicts the d*d block of
 the normalised exact correction t* = psi0 - (x.psi0) x, where psi0 is the
 ground state of the same Heff and x the current Davidson vector
 (harvest_pairs). All features are scale-free and local, so one network
-serves every chi. Inside Davidson the learned step is guarded: if it does
-not lower the residual norm it is reverted and taken with the diagonal
-preconditioner (mps.davidson's `fallback`).
+serves every chi.
+
+The network's output is added to a fixed skip step, by default the
+diagonal correction (r/delta)/rms(r/delta) ("diag"; "plain" uses r/rms(r),
+"none" nothing), and its last layer starts at zero: the untrained network
+is exactly the skip step. Two objectives train it:
+- train_sweep / rq_step: the Ritz value of the one-step Davidson subspace
+  span{x, t}, minimised directly over the chain's own Heff operators (its
+  gradient in t is 2 (b/|t_perp|) (H u - mu u), u = the new Ritz vector);
+- harvest_sweep + train: mean-squared error against the blocks of t*
+  (less the skip step).
+Inside Davidson the learned step is guarded: if it does not lower the
+residual norm it is reverted and taken with the diagonal preconditioner
+(mps.davidson's `fallback`).
 """
 import numpy as np
 
@@ -23,8 +34,13 @@ from .mps import davidson, diag_preconditioner
 class MLPPreconditioner:
     """Parameters, Adam state and the inference path of one network."""
 
-    def __init__(self, d, hidden=256, seed=0, floor=1e-8, chunk_rows=1 << 20):
+    SKIPS = ("none", "plain", "diag")
+
+    def __init__(self, d, hidden=256, seed=0, floor=1e-8, chunk_rows=1 << 20, skip="diag"):
+        if skip not in self.SKIPS:
+            raise ValueError("skip must be one of %s" % (self.SKIPS,))
         self.d = int(d)
+        self.skip = skip
         self.i, self.h, self.o = 3 * self.d * self.d, int(hidden), self.d * self.d
         self.floor = float(floor)
         self.chunk_rows = int(chunk_rows)
@@ -32,7 +48,10 @@ class MLPPreconditioner:
         rng = np.random.default_rng(seed)
         parts = []
         for fan_in, fan_out in ((self.i, self.h), (self.h, self.h), (self.h, self.o)):
-            parts.append(rng.standard_normal(fan_in * fan_out) / np.sqrt(fan_in))
+            w = rng.standard_normal(fan_in * fan_out) / np.sqrt(fan_in)
+            if fan_out == self.o and skip != "none":
+                w[:] = 0.0
+            parts.append(w)
             parts.append(np.zeros(fan_out))
         host = np.concatenate(parts).astype(np.float32)
         assert host.size == self.nparam
@@ -55,11 +74,12 @@ class MLPPreconditioner:
         return self
 
     def save(self, path):
-        np.savez(path, params=self.host_params(), d=self.d, hidden=self.h)
+        np.savez(path, params=self.host_params(), d=self.d, hidden=self.h, skip=self.skip)
 
     @staticmethod
     def load(path, **kw):
         z = np.load(path, allow_pickle=False)
+        kw.setdefault("skip", str(z["skip"]) if "skip" in z else "none")
         net = MLPPreconditioner(int(z["d"]), int(z["hidden"]), **kw)
         return net.set_params(z["params"])
 
@@ -103,7 +123,8 @@ class MLPPreconditioner:
 
     # ---------------------------------------------------------- inference
     def correction(self, q, lam, diag, xdims):
-        """t = network(features of q) scattered back to (chiL, d, d, chiR)."""
+        """t = network(features of q) scattered back to (chiL, d, d, chiR),
+        plus the skip step."""
         chiL, d, d2, chiR = xdims
         if d != self.d or d2 != self.d:
             raise _lib.DimensionMismatch("network built for d=%d, got %s" % (self.d, xdims))
@@ -117,7 +138,69 @@ class MLPPreconditioner:
                                    Y=self._buf("Y", n * self.o),
                                    acts=self._buf("acts", 4 * n * self.h))
             ops.mlp_scatter(Y, chiL, d, chiR, r0, n, t)
+        sv, sc = self.skip_step(q, lam, diag)
+        if sv is not None:
+            ops.axpby(sc, sv, 1.0, t)
         return t
+
+    def skip_step(self, q, lam, diag):
+        """(vector, scale) of the skip step: scale * vector has rms 1."""
+        if self.skip == "none":
+            return None, 0.0
+        sv = q if self.skip == "plain" else ops.precond_diag(q, diag, lam, self.floor)
+        sn = ops.nrm2(sv)
+        return sv, (np.sqrt(sv.numel()) / sn if sn > 0 else 0.0)
+
+    def rq_step(self, matvec, x, lam, q, diag, xdims, X=None, lr=1e-3):
+        """One Adam step on the Ritz value of span{x, t}, t = correction(q):
+        x a unit vector, lam = x.Hx, q = Hx - lam x. Returns mu - lam (the
+        energy the step gains, <= 0). X: the feature rows of q, if at hand."""
+        chiL, d, _, chiR = xdims
+        rows = chiL * chiR
+        if X is None:
+            X = ops.mlp_features(q, diag, lam, chiL, d, chiR, self.floor, 0, 1, rows,
+                                 X=self._buf("Xrq", rows * self.i))
+        Y, acts = ops.mlp_forward(self.params, X, rows, self.i, self.h, self.o,
+                                  Y=self._buf("Y", rows * self.o),
+                                  acts=self._buf("acts", 4 * rows * self.h))
+        t = self._buf("t", q.numel(), "float64")
+        ops.mlp_scatter(Y, chiL, d, chiR, 0, rows, t)
+        sv, sc = self.skip_step(q, lam, diag)
+        if sv is not None:
+            ops.axpby(sc, sv, 1.0, t)
+        Ht = matvec(t)
+        xt, tt, qt, tHt = ops.dot(x, t), ops.dot(t, t), ops.dot(q, t), ops.dot(t, Ht)
+        n2 = tt - xt * xt
+        if not n2 > 1e-24 * max(tt, 1e-300):
+            return 0.0
+        tn = np.sqrt(n2)
+        # the 2x2 projected problem on {x, e}, e = (t - (x.t) x)/|t_perp|
+        xHt = qt + lam * xt
+        ehe = (tHt - 2.0 * xt * xHt + xt * xt * lam) / n2
+        xhe = qt / tn
+        ev, U = np.linalg.eigh(np.array([[lam, xhe], [xhe, ehe]]))
+        mu, a, b = float(ev[0]), float(U[0, 0]), float(U[1, 0])
+        c = b / tn
+        ax = a - c * xt
+        # g = dmu/dt = 2c (Hu - mu u), u = ax x + c t, Hx = q + lam x
+        g = self._buf("g", q.numel(), "float64")
+        ops.axpby(c, Ht, 0.0, g)
+        ops.axpby(ax, q, 1.0, g)
+        ops.axpby(ax * (lam - mu), x, 1.0, g)
+        ops.axpby(-mu * c, t, 1.0, g)
+        G = ops.mlp_gather(g, 1.0, chiL, d, chiR, 0, 1, rows, T=self._buf("G", rows * self.o))
+        gn = ops.nrm2(G)
+        if not gn > 0:
+            return mu - lam
+        # mlp_backward differentiates mean((Y - T)^2): with T = Y - G/|G| the
+        # parameter gradient is the chain rule of dmu/dY, normalised per step
+        Tt = self._buf("Trq", rows * self.o)
+        ops.axpby(1.0, Y, 0.0, Tt)
+        ops.axpby(-1.0 / gn, G, 1.0, Tt)
+        ops.mlp_backward(self.params, X, Tt, acts, Y, rows, self.i, self.h, self.o, G=self.grad)
+        self.step += 1
+        ops.mlp_adam(self.params, self.grad, self.m, self.v, self.step, lr=lr)
+        return mu - lam
 
     def factory(self, guard=True, min_residual=0.0):
         """dmrg `precond` factory: (ProjMPO, b, xdims) -> precond(q, lam)
@@ -157,9 +240,11 @@ def exact_correction(matvec, x, dg, iters=40, errgoal=1e-10):
     return lam0, psi0, t
 
 
-def harvest_pairs(matvec, phi, dg, xdims, rows_per_bond, seed=0, floor=1e-8, iters=40):
+def harvest_pairs(matvec, phi, dg, xdims, rows_per_bond, seed=0, floor=1e-8, iters=40,
+                  skip="diag"):
     """Training rows from one Heff: features of the residual at x = phi/|phi|
-    and the matching blocks of t*/rms(t*). Rows are a strided subsample
+    and the matching blocks of t*/rms(t*), less the skip step (the target of
+    a network built with that skip). Rows are a strided subsample
     (row0 + k*step) of the chiL*chiR bond pairs. Returns (X, T, nrows, info)."""
     chiL, d, _, chiR = xdims
     x = ops.copy(phi)
@@ -177,6 +262,12 @@ def harvest_pairs(matvec, phi, dg, xdims, rows_per_bond, seed=0, floor=1e-8, ite
     tn = ops.nrm2(t)
     scale = np.sqrt(t.numel()) / tn if tn > 0 else 1.0
     T = ops.mlp_gather(t, scale, chiL, d, chiR, row0, step, n)
+    if skip != "none":
+        sv = r if skip == "plain" else ops.precond_diag(r, dg, lam, floor)
+        sn = ops.nrm2(sv)
+        if sn > 0:
+            ops.axpby(-1.0, ops.mlp_gather(sv, np.sqrt(sv.numel()) / sn, chiL, d, chiR, row0,
+                                           step, n), 1.0, T)
     return X, T, n, {"lambda": lam, "lambda0": lam0, "residual": ops.nrm2(r)}
 
 
@@ -213,13 +304,14 @@ def evaluate(matvec, phi, dg, xdims, net, maxiter=2, lam0=None):
 
 
 def harvest_sweep(H, psi, maxdim, rows_per_bond=4096, bonds=None, cutoff=1e-12, seed=0,
-                  iters=40, sweeps=1):
+                  iters=40, sweeps=1, skip="diag"):
     """Training rows from the chain's own effective Hamiltonians: `sweeps`
     full two-site sweeps (diagonal-preconditioned Davidson, 2 steps,
     replacebond at maxdim) that harvest pairs at the listed bonds of every
     left-to-right pass before each update, so the rows cover the states of a
     run from its random start towards convergence. psi (right-canonical,
-    centre at site 0) is left in the same form. Returns (X, T, rows, d)."""
+    centre at site 0) is left in the same form. `skip` as in harvest_pairs.
+    Returns (X, T, rows, d)."""
     from . import mps
     N = len(psi)
     PH = mps.ProjMPO(H)
@@ -239,7 +331,8 @@ def harvest_sweep(H, psi, maxdim, rows_per_bond=4096, bonds=None, cutoff=1e-12, 
             dg = PH.diag(b, pdims)
             if ha == 1 and b in bonds:
                 X, T, n, _ = harvest_pairs(mv, phi, dg, pdims, rows_per_bond,
-                                           seed=seed + 1000 * sw + b, iters=iters)
+                                           seed=seed + 1000 * sw + b, iters=iters,
+                                           skip=skip)
                 Xs.append(X)
                 Ts.append(T)
                 rows += n
@@ -255,3 +348,58 @@ def harvest_sweep(H, psi, maxdim, rows_per_bond=4096, bonds=None, cutoff=1e-12, 
                 PH.R[b + 1] = mps.update_right(R, Rd, psi.data[b + 1], psi.dims[b + 1],
                                                H.data[b + 1], H.dims[b + 1])
     return concat(Xs), concat(Ts), rows, d
+
+
+def train_sweep(net, H, psi, maxdim, steps_per_bond=10, cutoff=1e-12, sweeps=3, lr=1e-3,
+                both=True, log=None):
+    """Train `net` on the chain's own effective Hamiltonians by the one-step
+    Ritz value (MLPPreconditioner.rq_step): `sweeps` full two-site sweeps
+    (diagonal-preconditioned Davidson, 2 steps, replacebond at maxdim) that
+    take `steps_per_bond` Adam steps at every bond of the left-to-right
+    passes (and of the right-to-left ones with `both`) before its update,
+    so the network sees states from the random start towards convergence.
+    psi (right-canonical, centre at site 0) is left in the same form.
+    Returns the per-bond (first, last) energy gains of the learned step."""
+    from . import mps
+    N = len(psi)
+    PH = mps.ProjMPO(H)
+    PH.build_right(psi)
+    gains = []
+    for sw in range(sweeps):
+        for b, ha in mps.sweepnext(N):
+            A, Ad, B, Bd = psi.data[b], psi.dims[b], psi.data[b + 1], psi.dims[b + 1]
+            phi, pdims = ops.contract(A, Ad, [1, 2, 3], B, Bd, [3, 4, 5], [1, 2, 4, 5])
+            PH.prepare(b, pdims)
+
+            def mv(x, b=b, pdims=pdims):
+                return PH.product(b, x, pdims)
+
+            dg = PH.diag(b, pdims)
+            if (ha == 1 or both) and pdims[1] == net.d and pdims[2] == net.d:
+                x = ops.copy(phi)
+                ops.scal(x, 1.0 / ops.nrm2(x))
+                q = mv(x)
+                lam = ops.dot(x, q)
+                ops.axpby(-lam, x, 1.0, q)
+                rows = pdims[0] * pdims[3]
+                X = ops.mlp_features(q, dg, lam, pdims[0], net.d, pdims[3], net.floor, 0, 1, rows)
+                first = last = 0.0
+                for k in range(int(steps_per_bond)):
+                    last = net.rq_step(mv, x, lam, q, dg, pdims, X=X, lr=lr)
+                    if k == 0:
+                        first = last
+                gains.append((first, last))
+                if log is not None:
+                    log(sw, b, ha, first, last)
+            lam, phi = mps.davidson(mv, phi, maxiter=2, precond=mps.diag_preconditioner(dg))
+            ortho = "left" if ha == 1 else "right"
+            mps.replacebond(psi, b, phi, pdims, ortho, maxdim, 1, cutoff)
+            if ha == 1:
+                L, Ld = PH.L[b]
+                PH.L[b + 1] = mps.update_left(L, Ld, psi.data[b], psi.dims[b], H.data[b],
+                                              H.dims[b])
+            else:
+                R, Rd = PH.R[b + 2]
+                PH.R[b + 1] = mps.update_right(R, Rd, psi.data[b + 1], psi.dims[b + 1],
+                                               H.data[b + 1], H.dims[b + 1])
+    return gains
