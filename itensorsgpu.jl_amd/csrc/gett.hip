@@ -12,7 +12,8 @@
 // tensors into LDS, in whichever direction (m/n or k) is contiguous in HBM.
 //
 // Tile: 128x128 per 256-thread workgroup, 2x2 waves of 64x64, each wave a
-// 4x4 grid of 16x16 MFMA blocks.
+// 4x4 grid of 16x16 MFMA blocks (FP64 launches with fewer tiles than
+// workgroup slots: 64x64, 2x2 waves of 32x32).
 //   FP64   : v_mfma_f64_16x16x4_f64, FP64 accumulate          (BK = 16)
 //   FP32   : v_mfma_f32_16x16x4_f32, FP32 accumulate          (BK = 16)
 //   BF16x3 : a = ah + al, b = bh + bl (bf16, round-to-nearest-even);
@@ -46,12 +47,12 @@ __device__ __forceinline__ uint16_t f2bf_rne(float x) {
 }
 __device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float(((uint32_t)h) << 16); }
 
-template <typename T, int MODE>
+template <typename T, int MODE, int TS = 128>
 struct GettTraits {
   static constexpr bool BF = (MODE == ITG_MODE_BF16X3);
   static constexpr bool F64 = (MODE == ITG_MODE_NATIVE) && std::is_same<T, double>::value;
   using CT = typename std::conditional<F64, double, float>::type;
-  static constexpr int BM = 128, BN = 128;
+  static constexpr int BM = TS, BN = TS;   // 128, or 64 for FP64 launches with few tiles
   static constexpr int BKT = BF ? 32 : 16;
   static constexpr int NPT = BM * BKT / 256;   // elements per thread per operand
 };
@@ -72,11 +73,13 @@ __device__ __forceinline__ void tile_coords(int64_t tilesM, int64_t tilesN, int6
   tn = in / gsize;
 }
 
-template <typename T, int MODE, bool AK, bool BKF>
+template <typename T, int MODE, bool AK, bool BKF, int TS = 128>
 __global__ __launch_bounds__(256, 2) void gett_kernel(GettArgs g) {
-  using TR = GettTraits<T, MODE>;
+  using TR = GettTraits<T, MODE, TS>;
   using CT = typename TR::CT;
   constexpr int BM = TR::BM, BN = TR::BN, BKT = TR::BKT, NPT = TR::NPT;
+  // four waves in 2 x 2, each a WH x WH sub-tile of WT x WT 16 x 16 blocks
+  constexpr int WH = BM / 2, WT = BM / 32;
   constexpr bool BF = TR::BF;
   constexpr int PAD = BF ? 8 : 1;
 
@@ -197,11 +200,11 @@ __global__ __launch_bounds__(256, 2) void gett_kernel(GettArgs g) {
   };
 
   typedef typename std::conditional<TR::F64, d4_t, f4_t>::type acc_t;
-  acc_t acc[4][4];
+  acc_t acc[WT][WT];
 #pragma unroll
-  for (int i = 0; i < 4; ++i)
+  for (int i = 0; i < WT; ++i)
 #pragma unroll
-    for (int j = 0; j < 4; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
+    for (int j = 0; j < WT; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
 
   const int lr = lane & 15, lg = lane >> 4;
   if (kbeg < kend) load(kbeg);
@@ -211,20 +214,20 @@ __global__ __launch_bounds__(256, 2) void gett_kernel(GettArgs g) {
     __syncthreads();
     if (k0 + BKT < kend) load(k0 + BKT);
     if constexpr (BF) {
-      bf8_t ah[4], al[4], bh[4], bl[4];
+      bf8_t ah[WT], al[WT], bh[WT], bl[WT];
 #pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        int m = wm * 64 + i * 16 + lr;
+      for (int i = 0; i < WT; ++i) {
+        int m = wm * WH + i * 16 + lr;
         ah[i] = *(const bf8_t*)&Ah[m * (BKT + PAD) + lg * 8];
         al[i] = *(const bf8_t*)&Al[m * (BKT + PAD) + lg * 8];
-        int n = wn * 64 + i * 16 + lr;
+        int n = wn * WH + i * 16 + lr;
         bh[i] = *(const bf8_t*)&Bh[n * (BKT + PAD) + lg * 8];
         bl[i] = *(const bf8_t*)&Bl[n * (BKT + PAD) + lg * 8];
       }
 #pragma unroll
-      for (int i = 0; i < 4; ++i)
+      for (int i = 0; i < WT; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
+        for (int j = 0; j < WT; ++j) {
           if constexpr (BF) {
             acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[i], bh[j], acc[i][j], 0, 0, 0);
             acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bl[j], acc[i][j], 0, 0, 0);
@@ -234,16 +237,16 @@ __global__ __launch_bounds__(256, 2) void gett_kernel(GettArgs g) {
     } else {
 #pragma unroll
       for (int kk = 0; kk < BKT / 4; ++kk) {
-        CT a[4], b[4];
+        CT a[WT], b[WT];
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-          a[i] = As[(kk * 4 + lg) * (BM + PAD) + wm * 64 + i * 16 + lr];
-          b[i] = Bs[(kk * 4 + lg) * (BN + PAD) + wn * 64 + i * 16 + lr];
+        for (int i = 0; i < WT; ++i) {
+          a[i] = As[(kk * 4 + lg) * (BM + PAD) + wm * WH + i * 16 + lr];
+          b[i] = Bs[(kk * 4 + lg) * (BN + PAD) + wn * WH + i * 16 + lr];
         }
 #pragma unroll
-        for (int i = 0; i < 4; ++i)
+        for (int i = 0; i < WT; ++i)
 #pragma unroll
-          for (int j = 0; j < 4; ++j) {
+          for (int j = 0; j < WT; ++j) {
             if constexpr (TR::F64)
               acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[j], acc[i][j], 0, 0, 0);
             else if constexpr (!BF)
@@ -259,30 +262,83 @@ __global__ __launch_bounds__(256, 2) void gett_kernel(GettArgs g) {
   if (g.partial) {
     CT* P = (CT*)g.partial + (int64_t)blockIdx.z * g.M * g.N;
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+    for (int i = 0; i < WT; ++i)
 #pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        int64_t n = n0 + wn * 64 + j * 16 + lr;
+      for (int j = 0; j < WT; ++j) {
+        int64_t n = n0 + wn * WH + j * 16 + lr;
         if (n >= g.N) continue;
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-          int64_t m = m0 + wm * 64 + i * 16 + (TR::F64 ? 4 * r + lg : 4 * lg + r);
+          int64_t m = m0 + wm * WH + i * 16 + (TR::F64 ? 4 * r + lg : 4 * lg + r);
           if (m < g.M) P[n * g.M + m] = acc[i][j][r];
         }
       }
     return;
   }
   T* C = (T*)g.C;
+  if (g.sCm == 1 && g.sCn >= 0 && m0 + BM <= g.M && n0 + BN <= g.N) {
+    // interior tile of a column-major C: one column pointer per j and
+    // compile-time row offsets, no per-element bounds or offset arithmetic
+    // (a short epilogue: this code runs once per tile, from a cold
+    // instruction cache on small launches)
+    T* cb = C + (n0 + wn * WH + lr) * g.sCn + m0 + wm * WH + (TR::F64 ? lg : 4 * lg);
+    const double al = g.alpha, be = g.beta;
+    if (be != 0.0) {
 #pragma unroll
-  for (int i = 0; i < 4; ++i)
+      for (int j = 0; j < WT; ++j) {
+        T* cj = cb + (int64_t)(j * 16) * g.sCn;
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      int64_t n = n0 + wn * 64 + j * 16 + lr;
+        for (int i = 0; i < WT; ++i)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const int mo = i * 16 + (TR::F64 ? 4 * r : r);
+            cj[mo] = (T)(al * (double)acc[i][j][r] + be * (double)cj[mo]);
+          }
+      }
+    } else {
+#pragma unroll
+      for (int j = 0; j < WT; ++j) {
+        T* cj = cb + (int64_t)(j * 16) * g.sCn;
+#pragma unroll
+        for (int i = 0; i < WT; ++i)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) cj[i * 16 + (TR::F64 ? 4 * r : r)] =
+              (T)(al * (double)acc[i][j][r]);
+      }
+    }
+    return;
+  }
+  if (g.sCn == 1 && g.sCm >= 0 && m0 + BM <= g.M && n0 + BN <= g.N) {
+    // the same for a row-major C (itg_gemm's C^T): one row pointer per
+    // (i, r), compile-time column offsets
+    T* cb = C + (m0 + wm * WH + (TR::F64 ? lg : 4 * lg)) * g.sCm + n0 + wn * WH + lr;
+    const double al = g.alpha, be = g.beta;
+#pragma unroll
+    for (int i = 0; i < WT; ++i)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        T* ci = cb + (int64_t)(i * 16 + (TR::F64 ? 4 * r : r)) * g.sCm;
+        if (be != 0.0) {
+#pragma unroll
+          for (int j = 0; j < WT; ++j)
+            ci[j * 16] = (T)(al * (double)acc[i][j][r] + be * (double)ci[j * 16]);
+        } else {
+#pragma unroll
+          for (int j = 0; j < WT; ++j) ci[j * 16] = (T)(al * (double)acc[i][j][r]);
+        }
+      }
+    return;
+  }
+#pragma unroll
+  for (int i = 0; i < WT; ++i)
+#pragma unroll
+    for (int j = 0; j < WT; ++j) {
+      int64_t n = n0 + wn * WH + j * 16 + lr;
       if (n >= g.N) continue;
       int64_t on = goff(g.tCn, g.sCn, n);
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        int64_t m = m0 + wm * 64 + i * 16 + (TR::F64 ? 4 * r + lg : 4 * lg + r);
+        int64_t m = m0 + wm * WH + i * 16 + (TR::F64 ? 4 * r + lg : 4 * lg + r);
         if (m >= g.M) continue;
         int64_t o = on + goff(g.tCm, g.sCm, m);
         double v = g.alpha * (double)acc[i][j][r];
@@ -530,14 +586,22 @@ static int build_plan(itg_ctx* ctx, const std::string& key, int rA, const int64_
   return ITG_OK;
 }
 
-template <typename T, int MODE>
+template <typename T, int MODE, int TS = 128>
 static int launch_gett_t(itg_ctx* ctx, GettArgs& g, bool ak, bool bk) {
-  using TR = GettTraits<T, MODE>;
+  using TR = GettTraits<T, MODE, TS>;
   using CT = typename TR::CT;
   g.tilesM = (g.M + TR::BM - 1) / TR::BM;
   g.tilesN = (g.N + TR::BN - 1) / TR::BN;
   int64_t tiles = g.tilesM * g.tilesN;
   if (tiles > 0x7fffffff) return itg_fail(ctx, ITG_EUNSUPPORTED, "contraction too large");
+  if constexpr (TR::F64 && TS == 128) {
+    // FP64 with fewer 128 x 128 tiles than workgroup slots, or a short K:
+    // 64 x 64 tiles give each wave a quarter of the MFMA chain per k step
+    // and four times the workgroups, more of them resident per CU (the
+    // K = 64 trailing updates of the eigensolver)
+    if (tiles < 2 * (int64_t)std::max(ctx->num_cu, 1) || g.K <= 128)
+      return launch_gett_t<T, MODE, 64>(ctx, g, ak, bk);
+  }
   // split K when the output has too few tiles to fill the chip
   int nsplit = 1;
   int64_t target = 2 * (int64_t)std::max(ctx->num_cu, 1);
@@ -561,10 +625,10 @@ static int launch_gett_t(itg_ctx* ctx, GettArgs& g, bool ak, bool bk) {
   }
   dim3 grid((unsigned)tiles, 1, (unsigned)nsplit);
   int tk = itg_tstart(ctx, ITG_K_GETT);
-  if (ak && bk) gett_kernel<T, MODE, true, true><<<grid, 256, 0, ctx->stream>>>(g);
-  else if (ak) gett_kernel<T, MODE, true, false><<<grid, 256, 0, ctx->stream>>>(g);
-  else if (bk) gett_kernel<T, MODE, false, true><<<grid, 256, 0, ctx->stream>>>(g);
-  else gett_kernel<T, MODE, false, false><<<grid, 256, 0, ctx->stream>>>(g);
+  if (ak && bk) gett_kernel<T, MODE, true, true, TS><<<grid, 256, 0, ctx->stream>>>(g);
+  else if (ak) gett_kernel<T, MODE, true, false, TS><<<grid, 256, 0, ctx->stream>>>(g);
+  else if (bk) gett_kernel<T, MODE, false, true, TS><<<grid, 256, 0, ctx->stream>>>(g);
+  else gett_kernel<T, MODE, false, false, TS><<<grid, 256, 0, ctx->stream>>>(g);
   ITG_LAUNCH_CHECK(ctx);
   itg_tstop(ctx, tk, ITG_K_GETT, 2.0 * g.M * g.N * g.K,
             (double)sizeof(T) * (g.M * g.K + g.K * g.N + g.M * g.N * (g.beta != 0.0 ? 2 : 1)));
