@@ -226,11 +226,6 @@ __global__ __launch_bounds__(256) void trd_symv_tiles_kernel(
   }
 }
 
-// rows [64 b, 64 b + 64): w = tau (y - V a - W b) + alpha v, v -> A(:, j),
-// next panel column update and its squared-tail partial sum. Lane = row; the
-// four waves split the panel index q (q = wave mod 4) and meet in LDS once;
-// wave-level sums (identical in every wave) give y^T v, a^T b and w(j+1)
-// without block barriers.
 // sum of p[b] over b = t (mod 256) for thread t, eight loads in flight
 __device__ __forceinline__ double strided_partial(const double* __restrict__ p, int np, int t) {
   double q0 = 0.0, q1 = 0.0, q2 = 0.0, q3 = 0.0, q4 = 0.0, q5 = 0.0, q6 = 0.0, q7 = 0.0;
@@ -250,101 +245,100 @@ __device__ __forceinline__ double strided_partial(const double* __restrict__ p, 
 }
 
 
+constexpr int POST_ROWS = 16;   // rows per block of the post kernel
+
+// Rows [r0, r0 + 16), r0 = 16 (b0 + blockIdx.x), b0 = (j + 1) / 16 (rows up
+// to j are finished): w = tau (y - V a - W b) + alpha v, v -> A(:, j), the
+// next panel column's update and its squared-tail partial sum. Thread
+// (row rr = t % 16, part = t / 16) sums its share t = part (mod 16) of the
+// symv row partials and q = part (mod 16) of the panel terms; the 16 parts
+// meet in LDS once. The scalars (y^T v, a^T b, y(j+1), the panel dot at
+// j+1) are formed the same way in every block, so all blocks agree bit for
+// bit.
 __global__ __launch_bounds__(256) void trd_post_kernel(
     double* __restrict__ A, int n, int j, int i, int nsym, const double* __restrict__ Vp,
     double* __restrict__ Wp, const double* __restrict__ Ypart, int T,
     const double* __restrict__ ab, const double* __restrict__ pyv,
-    const double* __restrict__ tau, int next, double* __restrict__ pnorm) {
-  __shared__ double Vs[NB][TRD_ROWS + 1];
-  __shared__ double Ws[NB][TRD_ROWS + 1];
-  __shared__ double abs_[2 * NB];
-  __shared__ double xs[2 * NB];
-  __shared__ double pw[4][TRD_ROWS];
-  __shared__ double pn[4][TRD_ROWS];
+    const double* __restrict__ tau, int next, double* __restrict__ pnorm, int b0) {
+  __shared__ double red[3][POST_ROWS][POST_ROWS + 1];
+  __shared__ double sc[4][4];
   const size_t ld = (size_t)n;
   const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
-  const int r0 = blockIdx.x * TRD_ROWS;
-  const int r = r0 + lane;
+  const int rr = t % POST_ROWS, part = t / POST_ROWS;
+  const int r = (b0 + (int)blockIdx.x) * POST_ROWS + rr;
   const int jn = j + 1;
-  const double tj = tau[j];
-  // per-row values of wave 0, loaded early
   const int m = n - j - 1;
-  // y(r) = sum_t Ypart[t][r - jn]: wave w sums t = w (mod 4) for row `lane`;
-  // y(jn) with one t per lane, summed in every wave
-  __shared__ double ys[4][TRD_ROWS];
+  const bool live = r > j && r < n;   // rows of v, w and the update
+  // symv row partials: y(r) = sum_t Ypart[t][r - jn]
   double yp = 0.0;
-  if (r > j && r < n) {
-    // eight independent loads in flight per lane
+  if (live) {
     const double* yc = Ypart + (r - jn);
-    // up to 16 independent loads in flight per lane (T <= 64 in one round)
-    double q[16];
-#pragma unroll
-    for (int u = 0; u < 16; ++u) {
-      int tt = wv + 4 * u;
-      q[u] = tt < T ? yc[(size_t)tt * m] : 0.0;
+    double q0 = 0.0, q1 = 0.0, q2 = 0.0, q3 = 0.0;
+    int tt = part;
+    for (; tt + 3 * POST_ROWS < T; tt += 4 * POST_ROWS) {
+      q0 += yc[(size_t)tt * m];
+      q1 += yc[(size_t)(tt + POST_ROWS) * m];
+      q2 += yc[(size_t)(tt + 2 * POST_ROWS) * m];
+      q3 += yc[(size_t)(tt + 3 * POST_ROWS) * m];
     }
-    for (int tt = wv + 64; tt < T; tt += 4) q[0] += yc[(size_t)tt * m];
-#pragma unroll
-    for (int u = 8; u > 0; u >>= 1)
-#pragma unroll
-      for (int v = 0; v < u; ++v) q[v] += q[v + u];
-    yp = q[0];
+    for (; tt < T; tt += POST_ROWS) q0 += yc[(size_t)tt * m];
+    yp = (q0 + q1) + (q2 + q3);
   }
-  ys[wv][lane] = yp;
-  double y0 = 0.0;
-  for (int tt = lane; tt < T; tt += 64) y0 += Ypart[(size_t)tt * m];
-  y0 = wave_sum(y0);
-  double anr = 0.0;
-  if (wv == 0 && r < n && next && r >= jn) anr = A[r + (size_t)jn * ld];
-  __shared__ double yv4[4];
-  {
-    double pv = wave_sum(strided_partial(pyv, nsym, t));
-    if (lane == 0) yv4[wv] = pv;
-  }
-  if (t < 2 * i) abs_[t] = ab[t];
-  // stage V(:, 0..i) and W(:, 0..i-1) for this block's rows
-  for (int x = t; x < TRD_ROWS * (i + 1); x += 256) {
-    int rr = x % TRD_ROWS, q = x / TRD_ROWS, rx = r0 + rr;
-    Vs[q][rr] = rx < n ? Vp[rx + (size_t)q * ld] : 0.0;
-    Ws[q][rr] = (rx < n && q < i) ? Wp[rx + (size_t)q * ld] : 0.0;
-  }
-  if (next && t < i) {
-    xs[t] = Vp[jn + (size_t)t * ld];
-    xs[NB + t] = Wp[jn + (size_t)t * ld];
-  }
-  __syncthreads();
-  const double ytv = (yv4[0] + yv4[1]) + (yv4[2] + yv4[3]);
-  // a^T b and the dot for w(jn): one term per lane, summed in each wave
-  double atb = wave_sum(lane < i ? abs_[lane] * abs_[i + lane] : 0.0);
-  const double alpha = -0.5 * tj * tj * (ytv - 2.0 * atb);
-  double djn = 0.0;
-  if (next) djn = wave_sum(lane < i ? xs[lane] * abs_[lane] + xs[NB + lane] * abs_[i + lane] : 0.0);
-  // partial sums over q = wv (mod 4) for row `lane`
+  // panel terms over q < i: V(r,q) a_q + W(r,q) b_q and, for the next
+  // column, V(r,q) W(jn,q) + W(r,q) V(jn,q)
   double sw = 0.0, sn = 0.0;
-  for (int q = wv; q < i; q += 4) {
-    double vq = Vs[q][lane], wq = Ws[q][lane];
-    sw += vq * abs_[q] + wq * abs_[i + q];
-    sn += vq * xs[NB + q] + wq * xs[q];
+  if (live) {
+    for (int q = part; q < i; q += POST_ROWS) {
+      const double vq = Vp[r + (size_t)q * ld], wq = Wp[r + (size_t)q * ld];
+      sw += vq * ab[q] + wq * ab[i + q];
+      if (next) sn += vq * Wp[jn + (size_t)q * ld] + wq * Vp[jn + (size_t)q * ld];
+    }
   }
-  pw[wv][lane] = sw;
-  pn[wv][lane] = sn;
+  // block scalars: every thread a strided share of the v^T A22 v partials;
+  // wave 1: y(jn); wave 2: a^T b; wave 3: the panel dot at row jn
+  double pv = strided_partial(pyv, nsym, t);
+  double s1 = 0.0;
+  if (wv == 1) {
+    for (int tt = lane; tt < T; tt += 64) s1 += Ypart[(size_t)tt * m];
+  } else if (wv == 2) {
+    if (lane < i) s1 = ab[lane] * ab[i + lane];
+  } else if (wv == 3) {
+    if (next && lane < i)
+      s1 = Vp[jn + (size_t)lane * ld] * ab[lane] + Wp[jn + (size_t)lane * ld] * ab[i + lane];
+  }
+  pv = wave_sum(pv);
+  s1 = wave_sum(s1);
+  red[0][part][rr] = yp;
+  red[1][part][rr] = sw;
+  red[2][part][rr] = sn;
+  if (lane == 0) {
+    sc[0][wv] = pv;
+    sc[1][wv] = s1;
+  }
   __syncthreads();
   if (wv != 0) return;
-  const double yr = (ys[0][lane] + ys[1][lane]) + (ys[2][lane] + ys[3][lane]);
+  const double tj = tau[j];
+  const double ytv = (sc[0][0] + sc[0][1]) + (sc[0][2] + sc[0][3]);
+  const double y0 = sc[1][1], atb = sc[1][2], djn = sc[1][3];
+  const double alpha = -0.5 * tj * tj * (ytv - 2.0 * atb);
   double s = 0.0;
-  if (r < n) {
-    double vr = Vs[i][lane];
-    double wr = 0.0;
-    if (r > j && tj != 0.0) {
-      double acc = yr - ((pw[0][lane] + pw[1][lane]) + (pw[2][lane] + pw[3][lane]));
-      wr = tj * acc + alpha * vr;
+  if (lane < POST_ROWS && r < n) {
+    double yr = 0.0, swr = 0.0, snr = 0.0;
+#pragma unroll
+    for (int p = 0; p < POST_ROWS; ++p) {
+      yr += red[0][p][lane];
+      swr += red[1][p][lane];
+      snr += red[2][p][lane];
     }
+    const double vr = Vp[r + (size_t)i * ld];
+    double wr = 0.0;
+    if (r > j && tj != 0.0) wr = tj * (yr - swr) + alpha * vr;
     Wp[r + (size_t)i * ld] = wr;
     if (r > j) A[r + (size_t)j * ld] = vr;
     if (next && r >= jn) {
       // w(jn) for the panel column i (V(jn, i) = 1)
-      double wj1 = (tj != 0.0) ? tj * (y0 - djn) + alpha : 0.0;
-      double acc = anr - ((pn[0][lane] + pn[1][lane]) + (pn[2][lane] + pn[3][lane]));
+      const double wj1 = (tj != 0.0) ? tj * (y0 - djn) + alpha : 0.0;
+      double acc = A[r + (size_t)jn * ld] - snr;
       acc -= vr * wj1 + wr;
       A[r + (size_t)jn * ld] = acc;
       if (r >= jn + 2) s = acc * acc;
@@ -352,7 +346,7 @@ __global__ __launch_bounds__(256) void trd_post_kernel(
   }
   if (next) {
     s = wave_sum(s);
-    if (lane == 0) pnorm[blockIdx.x] = s;
+    if (lane == 0) pnorm[b0 + blockIdx.x] = s;
   }
 }
 
@@ -880,6 +874,7 @@ static int tridiagonalize(itg_ctx* ctx, int n, double* A, double* d, double* e, 
   for (int p = 0; p < n; p += NB) {
     int nbp = std::min(NB, n - p);
     const int npn = (n + TRD_ROWS - 1) / TRD_ROWS;
+    int pn_off = 0, pn_cnt = npn;
     if (p < n - 1) {
       trd_colnorm_kernel<<<npn, 256, 0, ctx->stream>>>(A, n, p, pnorm);
       ITG_LAUNCH_CHECK(ctx);
@@ -895,18 +890,23 @@ static int tridiagonalize(itg_ctx* ctx, int n, double* A, double* d, double* e, 
       int T = (m + TS - 1) / TS;
       int ntiles = T * (T + 1) / 2;
       int tk = itg_tstart(ctx, ITG_K_TRD_SYMV);
-      trd_symv_tiles_kernel<<<ntiles + i, 256, 0, ctx->stream>>>(A, n, j, i, pnorm, npn, Vp, Wp,
-                                                                 V2, Ypart, ab, pyv, ntiles, d, e,
-                                                                 tau);
+      trd_symv_tiles_kernel<<<ntiles + i, 256, 0, ctx->stream>>>(A, n, j, i, pnorm + pn_off,
+                                                                 pn_cnt, Vp, Wp, V2, Ypart, ab, pyv,
+                                                                 ntiles, d, e, tau);
       ITG_LAUNCH_CHECK(ctx);
       // lower triangle of A22 read once, the panel pairs over m rows
       itg_tstop(ctx, tk, ITG_K_TRD_SYMV, 2.0 * m * (m + 2.0 * i),
                 8.0 * (0.5 * m * (m + TS) + 2.0 * i * m + 2.0 * T * m));
       int next = (i + 1 < nbp) ? 1 : 0;
+      const int b0 = (j + 1) / POST_ROWS;
+      const int nblk = (n + POST_ROWS - 1) / POST_ROWS - b0;
       tk = itg_tstart(ctx, ITG_K_TRD_POST);
-      trd_post_kernel<<<npn, 256, 0, ctx->stream>>>(A, n, j, i, ntiles, Vp, Wp, Ypart, T, ab, pyv,
-                                                    tau, next, pnorm);
+      trd_post_kernel<<<nblk, 256, 0, ctx->stream>>>(A, n, j, i, ntiles, Vp, Wp, Ypart, T, ab, pyv,
+                                                     tau, next, pnorm, b0);
       ITG_LAUNCH_CHECK(ctx);
+      // the next column's reflector norm: partials of these blocks only
+      pn_off = b0;
+      pn_cnt = nblk;
       itg_tstop(ctx, tk, ITG_K_TRD_POST, 8.0 * m * (i + 1), 8.0 * m * (2.0 * i + 5 + T));
     }
     int q = p + nbp;
@@ -1088,7 +1088,8 @@ extern "C" int itg_eigh_trunc(itg_ctx* ctx, int dtype, int64_t n64, void* Ain, d
                  (2 * (size_t)BTG * NB * n + (size_t)BTG * NB * BTG * NB) * 8 +
                  (size_t)((n + NB - 1) / NB) * NB * NB * 8 +
                  (size_t)n * 8 * 26 + 4 * (size_t)n * NB * 8 + 4 * NB * NB * 8 + 64 * 1024 +
-                 (Tw * n + Tw * (Tw + 1) / 2 + 64) * 8 + 64 * 256;
+                 (Tw * n + Tw * (Tw + 1) / 2 + 64) * 8 + 64 * 256 +
+                 ((size_t)n / POST_ROWS + 1) * 8;
   void* ws;
   rc = eig_workspace(ctx, bytes, &ws);
   if (rc) return rc;
@@ -1103,7 +1104,7 @@ extern "C" int itg_eigh_trunc(itg_ctx* ctx, int dtype, int64_t n64, void* Ain, d
   double* VW = cv.take<double>(3 * (size_t)n * NB);
   double* Ypart = cv.take<double>(Tmax * n);
   double* ab = cv.take<double>(2 * NB);
-  double* pnorm = cv.take<double>((size_t)n / TRD_ROWS + 1);
+  double* pnorm = cv.take<double>((size_t)n / POST_ROWS + 1);
   double* pyv = cv.take<double>(Tmax * (Tmax + 1) / 2 + 64);
   double* d = cv.take<double>(n);
   double* e = cv.take<double>(n);
