@@ -586,6 +586,8 @@ struct DcBufs {
   int *cs, *gcol, *dcol, *rcol, *org, *meta;
 };
 
+constexpr int DC_ZLDS = 4096;   // largest merge whose z is staged in LDS
+
 __global__ __launch_bounds__(1024) void dc_prepare_kernel(int n, int s_lvl, int half,
                                                           const double* __restrict__ D,
                                                           const double* __restrict__ Q,
@@ -608,27 +610,41 @@ __global__ __launch_bounds__(1024) void dc_prepare_kernel(int n, int s_lvl, int 
   bitonic_sort(key, val, np2);
   double beta = e[r0 + s1 - 1];
   double sg = beta >= 0.0 ? 1.0 : -1.0;
+  // the sequential deflation pass reads d, z and the columns from LDS when
+  // z fits beside them (s_lvl <= DC_ZLDS), else z from global memory
+  double* z = s_lvl <= DC_ZLDS ? (double*)(val + np2) : b.zs + r0;
   for (int i = threadIdx.x; i < s; i += blockDim.x) {
     int c = val[i];
-    double z = (c < r0 + s1) ? Q[(size_t)c * n + (r0 + s1 - 1)] : sg * Q[(size_t)c * n + (r0 + s1)];
-    b.zs[r0 + i] = z * 0.7071067811865476;
-    b.ds[r0 + i] = key[i];
-    b.cs[r0 + i] = c;
+    double zi = (c < r0 + s1) ? Q[(size_t)c * n + (r0 + s1 - 1)] : sg * Q[(size_t)c * n + (r0 + s1)];
+    z[i] = zi * 0.7071067811865476;
   }
   __syncthreads();
+  __shared__ int ksh;
+  __shared__ double wsum[16];
   if (threadIdx.x == 0) {
     double rho = 2.0 * fabs(beta);
     int nd = 0, nrot = 0;
-    int k = dc_deflate(s, b.ds + r0, b.zs + r0, b.cs + r0, rho, b.dl + r0, b.wv + r0,
-                       b.gcol + r0, b.dval + r0, b.dcol + r0, &nd, b.rcol + 2 * r0,
-                       b.rcs + 2 * r0, &nrot);
-    double w2 = 0.0;
-    for (int j = 0; j < k; ++j) w2 += b.wv[r0 + j] * b.wv[r0 + j];
+    int k = dc_deflate(s, key, z, val, rho, b.dl + r0, b.wv + r0, b.gcol + r0, b.dval + r0,
+                       b.dcol + r0, &nd, b.rcol + 2 * r0, b.rcs + 2 * r0, &nrot);
+    ksh = k;
     b.meta[3 * m] = k;
     b.meta[3 * m + 1] = nd;
     b.meta[3 * m + 2] = nrot;
     b.mrho[2 * m] = rho;
-    b.mrho[2 * m + 1] = w2;
+  }
+  __syncthreads();
+  // sum of w^2 over the k kept entries: strided per thread, per wave, then
+  // the 16 wave sums in order
+  const int k = ksh;
+  double w2 = 0.0;
+  for (int j = threadIdx.x; j < k; j += blockDim.x) w2 += b.wv[r0 + j] * b.wv[r0 + j];
+  for (int o = 32; o > 0; o >>= 1) w2 += __shfl_xor(w2, o, 64);
+  if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = w2;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int q = 0; q < (int)(blockDim.x >> 6); ++q) t += wsum[q];
+    b.mrho[2 * m + 1] = t;
   }
 }
 
@@ -946,7 +962,8 @@ static int tridiag_dc(itg_ctx* ctx, int n, const double* d, const double* e, dou
   for (int half = 1; half < n; half *= 2) {
     int s_lvl = 2 * half;
     int nm = (n + s_lvl - 1) / s_lvl;
-    size_t lds = (size_t)s_lvl * (sizeof(double) + sizeof(int));
+    size_t lds = (size_t)s_lvl * (sizeof(double) + sizeof(int) +
+                                  (s_lvl <= DC_ZLDS ? sizeof(double) : 0));
     dc_prepare_kernel<<<nm, 1024, lds, ctx->stream>>>(n, s_lvl, half, D, Q, e, bufs);
     ITG_LAUNCH_CHECK(ctx);
     meta.assign(3 * nm, 0);
