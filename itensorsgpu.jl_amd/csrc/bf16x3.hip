@@ -182,25 +182,14 @@ __global__ __launch_bounds__(256, 2) void gemm_split_kernel(
         bh[i] = *(const bf8_t*)(sBh + nb_);
         bl[i] = *(const bf8_t*)(sBl + nb_);
       }
-      // each product over all 16 accumulators before the next: a dependent
-      // MFMA then issues 16 instructions after its predecessor (the
-      // 16x16x32 bf16 form needs about that many in flight per wave to run
-      // at full rate); the per-element order lo*hi, hi*lo, hi*hi is kept
 #pragma unroll
       for (int i = 0; i < 4; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j)
+        for (int j = 0; j < 4; ++j) {
           acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh[j], al[i], acc[i][j], 0, 0, 0);
-#pragma unroll
-      for (int i = 0; i < 4; ++i)
-#pragma unroll
-        for (int j = 0; j < 4; ++j)
           acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bl[j], ah[i], acc[i][j], 0, 0, 0);
-#pragma unroll
-      for (int i = 0; i < 4; ++i)
-#pragma unroll
-        for (int j = 0; j < 4; ++j)
           acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh[j], ah[i], acc[i][j], 0, 0, 0);
+        }
     }
   }
 
