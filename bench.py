@@ -204,6 +204,45 @@ def precond_ab(N, chi, sweeps, krylov, train_chi=64, steps_per_bond=10):
     return out
 
 
+def learned_form(N, chi, krylov, train_chi=64, steps_per_bond=10):
+    """The timed sweep in split-BF16 with the learned preconditioner: train
+    the guarded network by the one-step Ritz value over three sweeps of a
+    random chain at maxdim train_chi (seed 4), then time one sweep at maxdim
+    chi from the same random MPS (seed 1) as the plain split-BF16 sweep. The
+    network runs in the host-driven Davidson (2 steps per bond, diagonal
+    fallback); its training is reported, not timed with the sweep."""
+    import torch
+    from itg import _lib
+    from itg.mps import MPO, PhaseTimer, ProjMPO, Sweeps, dmrg, heisenberg_opsum, random_mps
+    from itg.precond import MLPPreconditioner, train_sweep
+    H = MPO.from_opsum(heisenberg_opsum([(j, j + 1) for j in range(N - 1)]), N)
+    t0 = time.perf_counter()
+    net = MLPPreconditioner(2, hidden=256, seed=1)
+    train_sweep(net, H, random_mps(N, 2, train_chi, seed=4), train_chi,
+                steps_per_bond=steps_per_bond, sweeps=3, lr=3e-4)
+    _lib.ctx().sync()
+    train_s = time.perf_counter() - t0
+    log("[learned] trained: %d Adam steps (%.1f s)" % (net.step, train_s))
+    psi = random_mps(N, 2, chi, seed=1)
+    ph = ProjMPO(H, "bf16x3")
+    ph.build_right(psi)
+    timer = PhaseTimer()
+    rec = []
+    torch.cuda.synchronize()
+    _lib.ctx().sync()
+    t1 = time.perf_counter()
+    E = dmrg(H, psi, Sweeps(1, maxdim=[chi], cutoff=[1e-12]), mode="bf16x3", krylovdim=krylov,
+             ph=ph, timer=timer, record=rec, precond=net.factory(), native=False,
+             sweep_callback=lambda r: log("[learned] timed", r))
+    torch.cuda.synchronize()
+    _lib.ctx().sync()
+    t2 = time.perf_counter()
+    return {"s_per_sweep": t2 - t1, "energy": E, "matvecs": rec[0]["matvecs"],
+            "reverted": rec[0]["reverted"], "phases": timer.report(),
+            "train": {"seconds": train_s, "adam_steps": net.step, "chi": train_chi,
+                      "objective": "one-step Ritz value"}}
+
+
 def cut_dims(N, d, chi):
     return [1] + [min(chi, d ** min(j, N - j)) for j in range(1, N)] + [1]
 
@@ -309,6 +348,8 @@ def main():
     ap.add_argument("--pc-chi", type=int, default=128,
                     help="maxdim of the converged precision check (0 = skip)")
     ap.add_argument("--pc-sweeps", type=int, default=10)
+    ap.add_argument("--no-learned", action="store_true",
+                    help="skip the split-BF16 + learned-preconditioner sweep")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: the sweep's Heff/eigensolver path runs on a "
@@ -346,6 +387,10 @@ def main():
     pc = None
     if len(modes) > 1 and args.pc_chi > 0:
         pc = precision_check(args.N, args.pc_chi, args.pc_sweeps, args.krylov, modes)
+    learned = None
+    if head == "bf16x3" and not args.no_learned and not args.config3:
+        learned = learned_form(args.N, args.chi, args.krylov)
+        log("learned form", learned)
     kern = r.get("kernels", {})
     mfma_kern = {k: v for k, v in kern.items()
                  if k != L_.KERNEL_NAMES[L_.K_WPASS] and v["launches"]}
@@ -423,6 +468,11 @@ def main():
                             "to %d, cutoff 1e-12; FP64 energies of the converged states"
                             % (args.pc_sweeps, args.pc_chi),
                 "energy": pc}
+    if learned is not None:
+        learned["energy_plain_sweep"] = r["energy"]
+        if len(modes) > 1:
+            learned["speedup_vs_" + modes[1]] = res[modes[1]]["s_per_sweep"] / learned["s_per_sweep"]
+        line["learned_form"] = learned
     if not args.no_cpu_baseline:
         cchi = min(args.cpu_chi, args.chi)
         t_bond = cpu_baseline(cchi)
