@@ -1219,21 +1219,22 @@ __global__ __launch_bounds__(1024) void qr_reflect_kernel(double* __restrict__ W
   for (int r = j + 1 + threadIdx.x; r < m; r += blockDim.x) a[r] *= scale;
 }
 
-// W[j:m, c] -= tau_j v (v^T W[j:m, c]) for c > j, one wave per column
-__global__ __launch_bounds__(256) void qr_apply_kernel(double* __restrict__ W, int m, int n, int j,
-                                                       const double* __restrict__ tau) {
-  int lane = threadIdx.x & 63;
-  int c = j + 1 + blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (c >= n) return;
-  double tj = tau[j];
+// W[j:m, c] -= tau_j v (v^T W[j:m, c]) for j < c < nend (the rest of the
+// panel), one 256-thread block per column
+__global__ __launch_bounds__(256) void qr_apply_kernel(double* __restrict__ W, int m, int nend,
+                                                       int j, const double* __restrict__ tau) {
+  __shared__ double sh4[4];
+  const int c = j + 1 + blockIdx.x;
+  if (c >= nend) return;
+  const double tj = tau[j];
   if (tj == 0.0) return;
   const double* v = W + (size_t)j * m;
   double* col = W + (size_t)c * m;
   double s = 0.0;
-  for (int r = j + lane; r < m; r += 64) s += (r == j ? 1.0 : v[r]) * col[r];
-  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
-  double f = tj * s;
-  for (int r = j + lane; r < m; r += 64) col[r] -= f * (r == j ? 1.0 : v[r]);
+  for (int r = j + threadIdx.x; r < m; r += 256) s += (r == j ? 1.0 : v[r]) * col[r];
+  s = block_sum256(s, sh4);
+  const double f = tj * s;
+  for (int r = j + threadIdx.x; r < m; r += 256) col[r] -= f * (r == j ? 1.0 : v[r]);
 }
 
 __global__ void qr_init_q_kernel(int m, int r, double* __restrict__ Q) {
@@ -1358,7 +1359,7 @@ extern "C" int itg_qr(itg_ctx* ctx, int dtype, int64_t m64, int64_t n64, const v
   if (m64 == 0 || n64 == 0) return ITG_OK;
   int m = (int)m64, n = (int)n64, r = std::min(m, n);
   bool f32 = dtype == ITG_F32;
-  size_t bytes = ((size_t)m * n + (size_t)m * r + (size_t)r * n + (size_t)m * NB + 2 * (size_t)NB * r +
+  size_t bytes = ((size_t)m * n + (size_t)m * r + (size_t)r * n + (size_t)m * NB + 2 * (size_t)NB * n +
                   2 * NB * NB + (size_t)r + 64) * sizeof(double) + 16 * 256;
   void* ws;
   int rc = eig_workspace(ctx, bytes, &ws);
@@ -1368,18 +1369,45 @@ extern "C" int itg_qr(itg_ctx* ctx, int dtype, int64_t m64, int64_t n64, const v
   double* Q = cv.take<double>((size_t)m * r);
   double* R = cv.take<double>((size_t)r * n);
   double* Vb = cv.take<double>((size_t)m * NB);
-  double* X = cv.take<double>((size_t)NB * r);
-  double* X2 = cv.take<double>((size_t)NB * r);
+  double* X = cv.take<double>((size_t)NB * n);
+  double* X2 = cv.take<double>((size_t)NB * n);
   double* Gm = cv.take<double>(NB * NB);
   double* Tm = cv.take<double>(NB * NB);
   double* tau = cv.take<double>(r);
   rc = itg_convert(ctx, (int64_t)m * n, dtype, Ain, ITG_F64, W);
   if (rc) return rc;
-  for (int j = 0; j < r; ++j) {
-    qr_reflect_kernel<<<1, 1024, 0, ctx->stream>>>(W, m, j, tau);
-    if (j < n - 1)
-      qr_apply_kernel<<<(n - j - 1 + 3) / 4, 256, 0, ctx->stream>>>(W, m, n, j, tau);
+  // blocked Householder QR (dgeqrf): each panel of NB columns is factored
+  // column by column within itself, then the trailing columns take the
+  // panel's block reflector, W2 -= V T^T (V^T W2), as GEMMs
+  for (int p = 0; p < r; p += NB) {
+    const int nbp = std::min(NB, r - p), pe = p + nbp;
+    for (int j = p; j < pe; ++j) {
+      qr_reflect_kernel<<<1, 1024, 0, ctx->stream>>>(W, m, j, tau);
+      if (j + 1 < pe)
+        qr_apply_kernel<<<pe - j - 1, 256, 0, ctx->stream>>>(W, m, pe, j, tau);
+      ITG_LAUNCH_CHECK(ctx);
+    }
+    const int nt = n - pe, m2 = m - p;
+    if (nt <= 0) continue;
+    bt_build_v_kernel<<<itg_blocks((int64_t)m * nbp, 256, 4096), 256, 0, ctx->stream>>>(W, m, p, nbp,
+                                                                                         0, Vb);
     ITG_LAUNCH_CHECK(ctx);
+    const double* V1 = Vb + p;
+    double* W2 = W + (size_t)pe * m + p;
+    rc = itg_gemm_internal(ctx, ITG_F64, ITG_MODE_NATIVE, 'T', 'N', nbp, nbp, m2, 1.0, V1, m, V1,
+                           m, 0.0, Gm, nbp);
+    if (rc) return rc;
+    bt_larft_kernel<<<1, 64, 0, ctx->stream>>>(nbp, Gm, tau + p, Tm);
+    ITG_LAUNCH_CHECK(ctx);
+    rc = itg_gemm_internal(ctx, ITG_F64, ITG_MODE_NATIVE, 'T', 'N', nbp, nt, m2, 1.0, V1, m, W2,
+                           m, 0.0, X, nbp);
+    if (rc) return rc;
+    rc = itg_gemm_internal(ctx, ITG_F64, ITG_MODE_NATIVE, 'T', 'N', nbp, nt, nbp, 1.0, Tm, nbp, X,
+                           nbp, 0.0, X2, nbp);
+    if (rc) return rc;
+    rc = itg_gemm_internal(ctx, ITG_F64, ITG_MODE_NATIVE, 'N', 'N', m2, nt, nbp, -1.0, V1, m, X2,
+                           nbp, 1.0, W2, m);
+    if (rc) return rc;
   }
   qr_init_q_kernel<<<itg_blocks((int64_t)m * r, 256, 8192), 256, 0, ctx->stream>>>(m, r, Q);
   ITG_LAUNCH_CHECK(ctx);
