@@ -619,10 +619,98 @@ __global__ __launch_bounds__(1024) void dc_prepare_kernel(int n, int s_lvl, int 
     z[i] = zi * 0.7071067811865476;
   }
   __syncthreads();
-  __shared__ int ksh;
-  __shared__ double wsum[16];
-  if (threadIdx.x == 0) {
-    double rho = 2.0 * fabs(beta);
+  __shared__ int ksh, serial, isum[16];
+  __shared__ double wsum[16], amax[2][16];
+  const double rho = 2.0 * fabs(beta);
+  if (s_lvl <= DC_ZLDS) {
+    // Without a Givens rotation the sequential deflation pass is a pure
+    // compaction: entries with rho |z| <= tol go to the deflated list, the
+    // rest to the kept list, both in sorted order. Every pair test of
+    // consecutive kept entries is made here with the original values; if
+    // none calls for a rotation the two lists are formed in parallel,
+    // else the sequential pass below runs as before.
+    int* kidx = (int*)(z + np2);
+    const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+    double dm = 0.0, zm = 0.0;
+    for (int i = t; i < s; i += blockDim.x) {
+      dm = fmax(dm, fabs(key[i]));
+      zm = fmax(zm, fabs(z[i]));
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+      dm = fmax(dm, __shfl_xor(dm, o, 64));
+      zm = fmax(zm, __shfl_xor(zm, o, 64));
+    }
+    if (lane == 0) {
+      amax[0][wv] = dm;
+      amax[1][wv] = zm;
+    }
+    if (t == 0) serial = 0;
+    __syncthreads();
+    dm = 0.0;
+    zm = 0.0;
+    for (int q = 0; q < (int)(blockDim.x >> 6); ++q) {
+      dm = fmax(dm, amax[0][q]);
+      zm = fmax(zm, amax[1][q]);
+    }
+    const double tol = 8.0 * ITG_DC_EPS * fmax(dm, zm);
+    // thread t owns the contiguous entries [t cs, t cs + cs)
+    const int cs = (s + (int)blockDim.x - 1) / (int)blockDim.x;
+    const int i0 = min(s, t * cs), i1 = min(s, i0 + cs);
+    int cnt = 0;
+    for (int i = i0; i < i1; ++i) cnt += (rho * fabs(z[i]) <= tol) ? 0 : 1;
+    // exclusive scan of the kept counts over the block
+    int inc = cnt;
+    for (int o = 1; o < 64; o <<= 1) {
+      int u = __shfl_up(inc, o, 64);
+      if (lane >= o) inc += u;
+    }
+    if (lane == 63) isum[wv] = inc;
+    __syncthreads();
+    int base = 0, total = 0;
+    for (int q = 0; q < (int)(blockDim.x >> 6); ++q) {
+      if (q < wv) base += isum[q];
+      total += isum[q];
+    }
+    int pos = base + inc - cnt;
+    for (int i = i0; i < i1; ++i) {
+      if (rho * fabs(z[i]) <= tol) {
+        b.dval[r0 + (i - pos)] = key[i];
+        b.dcol[r0 + (i - pos)] = val[i];
+      } else {
+        kidx[pos++] = i;
+      }
+    }
+    __syncthreads();
+    if (rho * zm <= tol) serial = 1;   // everything deflates: the sequential pass
+    for (int p = t + 1; p < total; p += blockDim.x) {
+      const int pj = kidx[p - 1], j = kidx[p];
+      double S = z[pj], Cc = z[j];
+      double tau = hypot(Cc, S);
+      double tt = key[j] - key[pj];
+      double c = Cc / tau, sn = -S / tau;
+      if (fabs(tt * c * sn) <= tol) serial = 1;
+    }
+    __syncthreads();
+    if (!serial) {
+      for (int p = t; p < total; p += blockDim.x) {
+        const int j = kidx[p];
+        b.dl[r0 + p] = key[j];
+        b.wv[r0 + p] = z[j];
+        b.gcol[r0 + p] = val[j];
+      }
+      if (t == 0) {
+        ksh = total;
+        b.meta[3 * m] = total;
+        b.meta[3 * m + 1] = s - total;
+        b.meta[3 * m + 2] = 0;
+        b.mrho[2 * m] = rho;
+      }
+    }
+  } else if (threadIdx.x == 0) {
+    serial = 1;
+  }
+  __syncthreads();
+  if (serial && threadIdx.x == 0) {
     int nd = 0, nrot = 0;
     int k = dc_deflate(s, key, z, val, rho, b.dl + r0, b.wv + r0, b.gcol + r0, b.dval + r0,
                        b.dcol + r0, &nd, b.rcol + 2 * r0, b.rcs + 2 * r0, &nrot);
@@ -963,7 +1051,7 @@ static int tridiag_dc(itg_ctx* ctx, int n, const double* d, const double* e, dou
     int s_lvl = 2 * half;
     int nm = (n + s_lvl - 1) / s_lvl;
     size_t lds = (size_t)s_lvl * (sizeof(double) + sizeof(int) +
-                                  (s_lvl <= DC_ZLDS ? sizeof(double) : 0));
+                                  (s_lvl <= DC_ZLDS ? sizeof(double) + sizeof(int) : 0));
     dc_prepare_kernel<<<nm, 1024, lds, ctx->stream>>>(n, s_lvl, half, D, Q, e, bufs);
     ITG_LAUNCH_CHECK(ctx);
     meta.assign(3 * nm, 0);

@@ -162,6 +162,27 @@ def test_eigh_degenerate_multiplets(dev):
     assert np.abs(Uh.T @ Uh - np.eye(k)).max() < 1e-12
 
 
+def test_eigh_degenerate_large(dev):
+    """Twenty 30-fold degenerate eigenvalues at n=600: the merges deflate by
+    Givens rotations (the sequential deflation pass), the spectrum and the
+    kept invariant subspaces must come out exact."""
+    rng = np.random.default_rng(17)
+    vals = 10.0 ** -np.linspace(0, 10, 20)
+    lam = np.repeat(vals, 30)
+    n = lam.size
+    Q, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    rho = (Q * lam) @ Q.T
+    w, U, k, te, dc = dev.eigh_trunc(up(dev, rho), n, maxdim=300, cutoff=0.0)
+    assert k == 300
+    wh = dev.to_host(w)[:k]
+    assert np.abs(wh - lam[:k]).max() <= 1e-14 * 10
+    Uh = ops.from_flat(dev.to_host(U)[: n * k], (n, k))
+    assert np.abs(Uh.T @ Uh - np.eye(k)).max() < 1e-12
+    P = Uh @ Uh.T
+    Pr = Q[:, :k] @ Q[:, :k].T
+    assert np.abs(P - Pr).max() < 1e-9
+
+
 def test_eigh_fp32_storage(dev):
     rng = np.random.default_rng(10)
     n = 150
