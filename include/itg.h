@@ -152,7 +152,10 @@ int itg_contract(itg_ctx* ctx, int dtype, int mode,
                  int rC, const int64_t* dimsC, const int32_t* labelsC, void* C,
                  double alpha, double beta);
 /* Plain column-major GEMM C(m,n) = alpha*op(A)*op(B) + beta*C, op = 'N'|'T'
- * (order-2 `*`, src/tensor/culinearalgebra.jl:9-17). */
+ * (order-2 `*`, src/tensor/culinearalgebra.jl:9-17). With A == B (same
+ * pointer and leading dimension), 'N','T' or 'T','N', m == n >= 256 and
+ * beta == 0 (a density matrix) only the lower triangle is multiplied and the
+ * upper one mirrored: C comes out exactly symmetric. */
 int itg_gemm(itg_ctx* ctx, int dtype, int mode, char transA, char transB,
              int64_t m, int64_t n, int64_t k, double alpha, const void* A,
              int64_t lda, const void* B, int64_t ldb, double beta, void* C,

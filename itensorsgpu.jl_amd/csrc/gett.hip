@@ -779,6 +779,25 @@ int itg_gemm_internal(itg_ctx* ctx, int dtype, int mode, char ta, char tb, int64
   return launch_gett(ctx, dtype, mode, g, ak, bk);
 }
 
+// C(r, c) = C(c, r) for r < c from the lower triangle, through 32 x 32 LDS
+// tiles (block (bi, bj) writes the upper tile at rows bi, columns bj >= bi)
+template <typename T>
+__global__ __launch_bounds__(256) void mirror_lower_kernel(int64_t n, T* __restrict__ C,
+                                                           int64_t ldc) {
+  __shared__ T t[32][33];
+  const int64_t bi = blockIdx.x, bj = blockIdx.y;
+  if (bj < bi) return;
+  for (int yy = threadIdx.y; yy < 32; yy += 8) {
+    int64_t r = bj * 32 + threadIdx.x, c = bi * 32 + yy;
+    if (r < n && c < n) t[yy][threadIdx.x] = C[r + c * ldc];
+  }
+  __syncthreads();
+  for (int yy = threadIdx.y; yy < 32; yy += 8) {
+    int64_t r = bi * 32 + threadIdx.x, c = bj * 32 + yy;
+    if (r < n && c < n && r < c) C[r + c * ldc] = t[threadIdx.x][yy];
+  }
+}
+
 extern "C" int itg_gemm(itg_ctx* ctx, int dtype, int mode, char ta, char tb, int64_t m, int64_t n,
                         int64_t k, double alpha, const void* A, int64_t lda, const void* B,
                         int64_t ldb, double beta, void* C, int64_t ldc) {
@@ -787,6 +806,24 @@ extern "C" int itg_gemm(itg_ctx* ctx, int dtype, int mode, char ta, char tb, int
   if (mode < 0 || mode > 2) return itg_fail(ctx, ITG_EARG, "unknown mode %d", mode);
   if (m < 0 || n < 0 || k < 0 || ldc < std::max<int64_t>(1, m))
     return itg_fail(ctx, ITG_EDIM, "bad GEMM dimensions");
+  const bool nt = (ta == 'N' || ta == 'n') && (tb == 'T' || tb == 't');
+  const bool tn = (ta == 'T' || ta == 't') && (tb == 'N' || tb == 'n');
+  if (A == B && lda == ldb && m == n && m >= 256 && k > 0 && beta == 0.0 && alpha != 0.0 &&
+      (nt || tn)) {
+    // C = alpha A A^T (or A^T A), the density matrix of a bond: the lower
+    // triangle (and one tile band) on MFMA, the rest mirrored, so C is
+    // exactly symmetric
+    int rc = itg_gemm_internal(ctx, dtype, mode, ta, tb, m, n, k, alpha, A, lda, B, ldb, beta, C,
+                               ldc, true);
+    if (rc) return rc;
+    dim3 grid((unsigned)((m + 31) / 32), (unsigned)((m + 31) / 32));
+    if (dtype == ITG_F64)
+      mirror_lower_kernel<double><<<grid, dim3(32, 8), 0, ctx->stream>>>(m, (double*)C, ldc);
+    else
+      mirror_lower_kernel<float><<<grid, dim3(32, 8), 0, ctx->stream>>>(m, (float*)C, ldc);
+    ITG_LAUNCH_CHECK(ctx);
+    return ITG_OK;
+  }
   return itg_gemm_internal(ctx, dtype, mode, ta, tb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
 }
 

@@ -271,3 +271,24 @@ def test_gemm_all_transposes(itg):
                      itg.to_device(ops.to_flat(B)), B.shape[0], 0.0, dC, m)
             ref = (A if ta == "N" else A.T) @ (B if tb == "N" else B.T)
             assert rel_err(ops.from_flat(itg.to_host(dC), (m, n)), ref) < 1e-13
+
+
+@pytest.mark.parametrize("m,k,ta,tb", [(256, 300, "N", "T"), (300, 257, "T", "N"),
+                                       (1000, 64, "N", "T")])
+def test_gemm_symmetric_product(itg, m, k, ta, tb):
+    """C = A A^T (or A^T A) with one buffer for both operands: the lower
+    triangle is multiplied and mirrored, so C is exactly symmetric and
+    matches the product to FP64 rounding; leading dimension larger than m."""
+    rng = np.random.default_rng(m + k)
+    lda = (m if ta == "N" else k) + 3
+    A = rng.standard_normal((lda, k if ta == "N" else m))
+    dA = itg.to_device(ops.to_flat(A))
+    ldc = m + 5
+    dC = itg.to_device(ops.to_flat(np.full((ldc, m), 7.0)))
+    itg.gemm(ta, tb, m, m, k, 0.5, dA, lda, dA, lda, 0.0, dC, ldc)
+    C = ops.from_flat(itg.to_host(dC), (ldc, m))
+    Av = A[:m, :] if ta == "N" else A[:k, :]
+    ref = 0.5 * (Av @ Av.T if ta == "N" else Av.T @ Av)
+    assert rel_err(C[:m], ref) < 1e-13
+    assert np.array_equal(C[:m], C[:m].T)
+    assert np.all(C[m:] == 7.0)
