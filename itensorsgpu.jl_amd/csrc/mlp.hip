@@ -22,8 +22,9 @@ __device__ __forceinline__ float gelu_grad_f(float x) {
 }
 
 // Z (f x rows) += b[f]; A = gelu(Z) (A may be null: bias only)
-__global__ void bias_act_kernel(int f, int64_t rows, float* __restrict__ Z, const float* __restrict__ b,
-                                float* __restrict__ A) {
+// (A may alias Z: each element is read once, then written)
+__global__ void bias_act_kernel(int f, int64_t rows, float* Z, const float* __restrict__ b,
+                                float* A) {
   int64_t n = (int64_t)f * rows;
   for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n;
        e += (int64_t)gridDim.x * blockDim.x) {
@@ -142,6 +143,32 @@ extern "C" int itg_mlp_forward(itg_ctx* ctx, int rows, int in, int h, int out, c
                          h, 0.0, Z2, h);
   if (rc) return rc;
   bias_act_kernel<<<itg_blocks(rh, 256, 8192), 256, 0, ctx->stream>>>(h, rows, Z2, P + L.b2, A2);
+  rc = itg_gemm_internal(ctx, ITG_F32, ITG_MODE_NATIVE, 'T', 'N', out, rows, h, 1.0, P + L.W3, h,
+                         A2, h, 0.0, Y, out);
+  if (rc) return rc;
+  bias_act_kernel<<<itg_blocks((int64_t)rows * out, 256, 8192), 256, 0, ctx->stream>>>(
+      out, rows, Y, P + L.b3, nullptr);
+  ITG_LAUNCH_CHECK(ctx);
+  return ITG_OK;
+}
+
+extern "C" int itg_mlp_infer(itg_ctx* ctx, int rows, int in, int h, int out, const float* P,
+                             const float* X, float* Y, float* work) {
+  ITG_CHECK_CTX(ctx);
+  int rc = check_shape(ctx, rows, in, h, out);
+  if (rc) return rc;
+  Layout L = layout(in, h, out);
+  int64_t rh = (int64_t)rows * h;
+  float* A1 = work;
+  float* A2 = work + rh;
+  rc = itg_gemm_internal(ctx, ITG_F32, ITG_MODE_NATIVE, 'T', 'N', h, rows, in, 1.0, P + L.W1, in,
+                         X, in, 0.0, A1, h);
+  if (rc) return rc;
+  bias_act_kernel<<<itg_blocks(rh, 256, 8192), 256, 0, ctx->stream>>>(h, rows, A1, P + L.b1, A1);
+  rc = itg_gemm_internal(ctx, ITG_F32, ITG_MODE_BF16X3, 'T', 'N', h, rows, h, 1.0, P + L.W2, h, A1,
+                         h, 0.0, A2, h);
+  if (rc) return rc;
+  bias_act_kernel<<<itg_blocks(rh, 256, 8192), 256, 0, ctx->stream>>>(h, rows, A2, P + L.b2, A2);
   rc = itg_gemm_internal(ctx, ITG_F32, ITG_MODE_NATIVE, 'T', 'N', out, rows, h, 1.0, P + L.W3, h,
                          A2, h, 0.0, Y, out);
   if (rc) return rc;

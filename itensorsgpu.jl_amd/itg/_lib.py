@@ -106,6 +106,8 @@ _SIGS = {
     "itg_mlp_param_count": (C.c_int64, [C.c_int, C.c_int, C.c_int]),
     "itg_mlp_forward": (C.c_int, [_ctxp, C.c_int, C.c_int, C.c_int, C.c_int, _vp, _vp, _vp,
                                   _vp]),
+    "itg_mlp_infer": (C.c_int, [_ctxp, C.c_int, C.c_int, C.c_int, C.c_int, _vp, _vp, _vp,
+                                  _vp]),
     "itg_mlp_backward": (C.c_int, [_ctxp, C.c_int, C.c_int, C.c_int, C.c_int, _vp, _vp, _vp,
                                    _vp, _vp, _vp, _dp]),
     "itg_mlp_adam": (C.c_int, [_ctxp, C.c_int64, _vp, _vp, _vp, _vp, C.c_int, C.c_float,

@@ -342,6 +342,19 @@ def mlp_forward(P, X, rows, i, h, o, Y=None, acts=None):
     return Y, acts
 
 
+def mlp_infer(P, X, rows, i, h, o, Y=None, work=None):
+    """Y = MLP(X) for inference (no saved activations, split-BF16 hidden
+    product); work: 2*rows*h floats."""
+    if Y is None:
+        Y = empty(rows * o, "float32")
+    if work is None:
+        work = empty(2 * rows * h, "float32")
+    c = ctx()
+    c.check(lib().itg_mlp_infer(c.ptr, int(rows), int(i), int(h), int(o), ptr(P), ptr(X), ptr(Y),
+                                ptr(work)))
+    return Y
+
+
 def mlp_backward(P, X, T, acts, Y, rows, i, h, o, G=None):
     """Gradient of mean((Y - T)^2) into G; returns (G, loss)."""
     if G is None:

@@ -36,7 +36,7 @@ class MLPPreconditioner:
 
     SKIPS = ("none", "plain", "diag")
 
-    def __init__(self, d, hidden=256, seed=0, floor=1e-8, chunk_rows=1 << 20, skip="diag"):
+    def __init__(self, d, hidden=256, seed=0, floor=1e-8, chunk_rows=1 << 22, skip="diag"):
         if skip not in self.SKIPS:
             raise ValueError("skip must be one of %s" % (self.SKIPS,))
         self.d = int(d)
@@ -134,9 +134,9 @@ class MLPPreconditioner:
             n = min(self.chunk_rows, rows - r0)
             X = ops.mlp_features(q, diag, lam, chiL, d, chiR, self.floor, r0, 1, n,
                                  X=self._buf("X", n * self.i))
-            Y, _ = ops.mlp_forward(self.params, X, n, self.i, self.h, self.o,
-                                   Y=self._buf("Y", n * self.o),
-                                   acts=self._buf("acts", 4 * n * self.h))
+            Y = ops.mlp_infer(self.params, X, n, self.i, self.h, self.o,
+                              Y=self._buf("Y", n * self.o),
+                              work=self._buf("work", 2 * n * self.h))
             ops.mlp_scatter(Y, chiL, d, chiR, r0, n, t)
         sv, sc = self.skip_step(q, lam, diag)
         if sv is not None:

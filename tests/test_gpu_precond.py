@@ -91,6 +91,22 @@ def test_features_gather_scatter_vs_oracle(itg, chiL, d, chiR, row0, step):
         ops.mlp_features(rd, dd, lam, chiL, d, chiR, 1e-8, 0, 1, chiL * chiR + 1)
 
 
+@pytest.mark.parametrize("rows,h", [(1000, 256), (37, 64)])
+def test_mlp_infer_matches_forward(itg, rows, h):
+    """The inference path (no saved activations, split-BF16 hidden product)
+    against the FP32 forward: within the split's ~1e-6 relative error."""
+    from itg import ops
+    rng = np.random.default_rng(rows)
+    i, o = 12, 4
+    n = ops.mlp_param_count(i, h, o)
+    P = ops.to_device((rng.standard_normal(n) * 0.2).astype(np.float32), "float32")
+    X = ops.to_device(rng.standard_normal(rows * i).astype(np.float32), "float32")
+    Yf, _ = ops.mlp_forward(P, X, rows, i, h, o)
+    Yi = ops.mlp_infer(P, X, rows, i, h, o)
+    a, b = ops.to_host(Yi), ops.to_host(Yf)
+    assert np.linalg.norm(a - b) <= 2e-5 * np.linalg.norm(b)
+
+
 def test_mlp_training_lowers_loss(itg):
     from itg import ops
     from itg.precond import MLPPreconditioner
