@@ -265,11 +265,12 @@ int itg_heff2_diag(itg_ctx* ctx, int dtype, int64_t chiL, int64_t wl,
 int64_t itg_mlp_param_count(int in, int hidden, int out);
 int itg_mlp_forward(itg_ctx* ctx, int rows, int in, int hidden, int out,
                     const float* params, const float* X, float* Y, float* acts);
-/* Inference only: Y = MLP(X) with no saved activations. `work` holds
- * 2*rows*hidden floats. Bias and GELU are applied in place, and the
- * hidden x hidden product runs in split-BF16 (FP32 accumulation; ~1e-6
- * relative to the FP32 forward), for the preconditioner's many-row
- * evaluations. */
+/* Inference only: Y = MLP(X) with no saved activations, for the
+ * preconditioner's many-row evaluations. hidden == 256 with in <= 16 and
+ * out <= 4, or in <= 32 and out <= 9, run one fused kernel (activations stay on chip; the hidden x
+ * hidden product in bf16 with FP32 accumulation, ~3e-3 relative to the FP32
+ * forward) and leave `work` unused; other shapes use the forward's FP32
+ * products with `work` (2*rows*hidden floats) for the activations. */
 int itg_mlp_infer(itg_ctx* ctx, int rows, int in, int hidden, int out,
                   const float* params, const float* X, float* Y, float* work);
 /* Mean-squared-error loss against T; accumulates parameter gradients into

@@ -91,10 +91,11 @@ def test_features_gather_scatter_vs_oracle(itg, chiL, d, chiR, row0, step):
         ops.mlp_features(rd, dd, lam, chiL, d, chiR, 1e-8, 0, 1, chiL * chiR + 1)
 
 
-@pytest.mark.parametrize("rows,h", [(1000, 256), (37, 64)])
-def test_mlp_infer_matches_forward(itg, rows, h):
-    """The inference path (no saved activations, split-BF16 hidden product)
-    against the FP32 forward: within the split's ~1e-6 relative error."""
+@pytest.mark.parametrize("rows,h,tol", [(1000, 256, 1e-2), (64, 256, 1e-2), (37, 64, 2e-5)])
+def test_mlp_infer_matches_forward(itg, rows, h, tol):
+    """The inference path against the FP32 forward: h = 256 runs the fused
+    kernel (hidden product in bf16, ~3e-3 relative), other widths the
+    forward's FP32 products without saved activations."""
     from itg import ops
     rng = np.random.default_rng(rows)
     i, o = 12, 4
@@ -104,7 +105,7 @@ def test_mlp_infer_matches_forward(itg, rows, h):
     Yf, _ = ops.mlp_forward(P, X, rows, i, h, o)
     Yi = ops.mlp_infer(P, X, rows, i, h, o)
     a, b = ops.to_host(Yi), ops.to_host(Yf)
-    assert np.linalg.norm(a - b) <= 2e-5 * np.linalg.norm(b)
+    assert np.linalg.norm(a - b) <= tol * np.linalg.norm(b)
 
 
 def test_mlp_training_lowers_loss(itg):
