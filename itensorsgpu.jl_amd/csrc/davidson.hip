@@ -49,13 +49,18 @@ __global__ __launch_bounds__(256) void multidot_kernel(int64_t n, const double* 
         (sh[0][threadIdx.x] + sh[1][threadIdx.x]) + (sh[2][threadIdx.x] + sh[3][threadIdx.x]);
 }
 
-__global__ void multidot_sum_kernel(int nb, int cnt, const double* __restrict__ part,
-                                    double* __restrict__ out) {
-  int k = threadIdx.x;
-  if (k >= cnt) return;
-  double s = 0.0;
-  for (int b = 0; b < nb; ++b) s += part[(size_t)b * cnt + k];
-  out[k] = s;
+// out[k] = sum_b part[b][k]: wave w sums the dots k = w (mod 4), its lanes
+// striding over the block partials (a fixed order, so every call agrees)
+__global__ __launch_bounds__(256) void multidot_sum_kernel(int nb, int cnt,
+                                                           const double* __restrict__ part,
+                                                           double* __restrict__ out) {
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  for (int k = wv; k < cnt; k += 4) {
+    double s = 0.0;
+    for (int b = lane; b < nb; b += 64) s += part[(size_t)b * cnt + k];
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+    if (lane == 0) out[k] = s;
+  }
 }
 
 struct Coefs {
@@ -80,7 +85,7 @@ int multidot(itg_ctx* ctx, int64_t n, const double* V, int cnt, const double* x,
   unsigned nb = itg_blocks(n, 256, 512);
   double* part = ctx->red + 32;
   multidot_kernel<<<nb, 256, 0, ctx->stream>>>(n, V, cnt, x, part);
-  multidot_sum_kernel<<<1, 64, 0, ctx->stream>>>((int)nb, cnt, part, ctx->red);
+  multidot_sum_kernel<<<1, 256, 0, ctx->stream>>>((int)nb, cnt, part, ctx->red);
   ITG_LAUNCH_CHECK(ctx);
   ITG_HIP(ctx, hipMemcpyAsync(ctx->red_host, ctx->red, cnt * sizeof(double),
                               hipMemcpyDeviceToHost, ctx->stream));
