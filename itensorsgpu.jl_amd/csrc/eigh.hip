@@ -913,6 +913,18 @@ __global__ __launch_bounds__(1024) void dc_final_sort_kernel(int n, const double
   }
 }
 
+// out = min over p < k of perm[p] among those < kt (else kt), as a double
+__global__ __launch_bounds__(256) void dc_first_kept_kernel(const int* __restrict__ perm, int k,
+                                                            int kt, double* __restrict__ out) {
+  __shared__ int sh[4];
+  int v = kt;
+  for (int p = threadIdx.x; p < k; p += 256) v = min(v, perm[p] < kt ? perm[p] : kt);
+  for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, 64));
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) out[0] = (double)min(min(sh[0], sh[1]), min(sh[2], sh[3]));
+}
+
 __global__ void gather_cols_kernel(int n, int k, const double* __restrict__ Q,
                                    const int* __restrict__ perm, double* __restrict__ U) {
   int64_t tot = (int64_t)n * k;
@@ -1040,9 +1052,13 @@ static int tridiagonalize(itg_ctx* ctx, int n, double* A, double* d, double* e, 
 
 // Eigen-decomposition of the tridiagonal (d, e): eigenvalues D (unsorted)
 // and eigenvectors as the columns of the returned buffer (Qa or Qb).
+// With `top` non-null the last merge's eigenvector product is not launched
+// but returned in *top (top->M = 0 if there is none), for the caller to run
+// on the columns it keeps.
 static int tridiag_dc(itg_ctx* ctx, int n, const double* d, const double* e, double* Qa,
                       double* Qb, double* Da, double* Db, double* Vb, DcBufs bufs, int64_t* tab,
-                      double** Qout, double** Dout) {
+                      double** Qout, double** Dout, GettArgs* top) {
+  if (top) top->M = 0;
   dc_init_kernel<<<itg_blocks((int64_t)n * n, 256, 16384), 256, 0, ctx->stream>>>(n, d, e, Da, Qa, Qb);
   ITG_LAUNCH_CHECK(ctx);
   double *Q = Qa, *Qn = Qb, *D = Da, *Dn = Db;
@@ -1082,6 +1098,10 @@ static int tridiag_dc(itg_ctx* ctx, int n, const double* d, const double* e, dou
         g.sBk = -1; g.tBk = tab; g.sBn = 1;
         g.sCm = n; g.sCn = 1;
         g.alpha = 1.0; g.beta = 0.0;
+        if (top && s_lvl >= n) {
+          *top = g;   // the last merge (nm == 1, r0 == 0)
+          continue;
+        }
         int rc = itg_gett_launch(ctx, ITG_F64, ITG_MODE_NATIVE, &g, true, false);
         if (rc) return rc;
       }
@@ -1240,7 +1260,8 @@ extern "C" int itg_eigh_trunc(itg_ctx* ctx, int dtype, int64_t n64, void* Ain, d
   if (rc) return rc;
   // 2. divide and conquer
   double *Q, *D;
-  rc = tridiag_dc(ctx, n, d, e, Qa, Qb, Da, Db, Vbuf, b, tab, &Q, &D);
+  GettArgs top = {};
+  rc = tridiag_dc(ctx, n, d, e, Qa, Qb, Da, Db, Vbuf, b, tab, &Q, &D, &top);
   if (rc) return rc;
   // 3. sort descending, truncate
   {
@@ -1261,6 +1282,29 @@ extern "C" int itg_eigh_trunc(itg_ctx* ctx, int dtype, int64_t n64, void* Ain, d
     k = (int64_t)h[0];
     te = h[1];
     dc = h[2];
+  }
+  // the last merge's eigenvector product, for its non-deflated columns from
+  // the first one kept on (Q columns [0, top.M) are the non-deflated ones;
+  // their eigenvalues increase with the column, so the kept ones are a tail)
+  if (top.M > 0) {
+    int64_t j0 = 0;
+    if (k < n) {
+      dc_first_kept_kernel<<<1, 256, 0, ctx->stream>>>(perm, (int)k, (int)top.M, out3 + 3);
+      ITG_LAUNCH_CHECK(ctx);
+      double h0;
+      ITG_HIP(ctx, hipMemcpyAsync(&h0, out3 + 3, sizeof(double), hipMemcpyDeviceToHost,
+                                  ctx->stream));
+      ITG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+      j0 = (int64_t)h0;
+    }
+    if (j0 < top.M) {
+      GettArgs g = top;
+      g.A = (const double*)top.A + (size_t)j0 * top.sAm;
+      g.C = (double*)top.C + (size_t)j0 * top.sCm;
+      g.M = top.M - j0;
+      rc = itg_gett_launch(ctx, ITG_F64, ITG_MODE_NATIVE, &g, true, false);
+      if (rc) return rc;
+    }
   }
   // 4. kept eigenvectors: gather, back-transform
   gather_cols_kernel<<<itg_blocks((int64_t)n * k, 256, 16384), 256, 0, ctx->stream>>>(n, (int)k, Q, perm, U);
