@@ -269,8 +269,9 @@ int itg_mlp_forward(itg_ctx* ctx, int rows, int in, int hidden, int out,
  * preconditioner's many-row evaluations. hidden == 256 with in <= 16 and
  * out <= 4, or in <= 32 and out <= 9, run one fused kernel (activations stay on chip; the hidden x
  * hidden product in bf16 with FP32 accumulation, ~3e-3 relative to the FP32
- * forward) and leave `work` unused; other shapes use the forward's FP32
- * products with `work` (2*rows*hidden floats) for the activations. */
+ * forward; params 16-byte aligned) and leave `work` unused; other shapes
+ * use the forward's FP32 products with `work` (2*rows*hidden floats) for the
+ * activations. */
 int itg_mlp_infer(itg_ctx* ctx, int rows, int in, int hidden, int out,
                   const float* params, const float* X, float* Y, float* work);
 /* Mean-squared-error loss against T; accumulates parameter gradients into

@@ -282,13 +282,15 @@ extern "C" int itg_mlp_infer(itg_ctx* ctx, int rows, int in, int h, int out, con
   if (rc) return rc;
   Layout L = layout(in, h, out);
   const unsigned nblk = (unsigned)((rows + MI_RB - 1) / MI_RB);
-  if (h == MI_H && in <= 16 && out <= 4) {
+  // the fused kernel reads W2 in 16-byte vectors
+  const bool aligned = ((uintptr_t)(P + L.W2) & 15) == 0;
+  if (aligned && h == MI_H && in <= 16 && out <= 4) {
     mlp_infer_kernel<16, 4><<<nblk, 256, 0, ctx->stream>>>(
         rows, in, out, P + L.W1, P + L.b1, P + L.W2, P + L.b2, P + L.W3, P + L.b3, X, Y);
     ITG_LAUNCH_CHECK(ctx);
     return ITG_OK;
   }
-  if (h == MI_H && in <= 32 && out <= 9) {
+  if (aligned && h == MI_H && in <= 32 && out <= 9) {
     mlp_infer_kernel<32, 9><<<nblk, 256, 0, ctx->stream>>>(
         rows, in, out, P + L.W1, P + L.b1, P + L.W2, P + L.b2, P + L.W3, P + L.b3, X, Y);
     ITG_LAUNCH_CHECK(ctx);
