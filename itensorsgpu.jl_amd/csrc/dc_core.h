@@ -165,10 +165,13 @@ ITG_HD static inline int dc_deflate(int s, double* d, double* z, const int* col,
       continue;
     }
     if (pj < 0) { pj = j; continue; }
+    // |z| <= 1 here and kept entries have rho |z| > tol, so the plain norm
+    // neither overflows nor underflows
     double S = z[pj], Cc = z[j];
-    double tau = hypot(Cc, S);
+    double tau = sqrt(Cc * Cc + S * S);
     double t = d[j] - d[pj];
-    double c = Cc / tau, sn = -S / tau;
+    double it = 1.0 / tau;
+    double c = Cc * it, sn = -S * it;
     if (fabs(t * c * sn) <= tol) {
       z[j] = tau;
       z[pj] = 0.0;

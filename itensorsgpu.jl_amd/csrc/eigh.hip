@@ -685,9 +685,10 @@ __global__ __launch_bounds__(1024) void dc_prepare_kernel(int n, int s_lvl, int 
     for (int p = t + 1; p < total; p += blockDim.x) {
       const int pj = kidx[p - 1], j = kidx[p];
       double S = z[pj], Cc = z[j];
-      double tau = hypot(Cc, S);
+      double tau = sqrt(Cc * Cc + S * S);
       double tt = key[j] - key[pj];
-      double c = Cc / tau, sn = -S / tau;
+      double it = 1.0 / tau;
+      double c = Cc * it, sn = -S * it;
       if (fabs(tt * c * sn) <= tol) serial = 1;
     }
     __syncthreads();
