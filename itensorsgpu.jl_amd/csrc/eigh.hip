@@ -50,11 +50,12 @@ __device__ double block_sum(double v, double* sh) {
 
 // ------------------------------------------------------- tridiagonalisation
 // Per column j (panel column i) two kernels run across the whole GPU:
-//  trd_reflect_symv: every block forms the reflector of column j from the
+//  trd_symv_tiles: every block forms the reflector of column j from the
 //    norm partials left by the previous kernel (identically in every block);
-//    its waves compute y = A22 v (one wave per trailing column, from the
-//    unscaled column and the scale), the panel dots a = W^T v and
-//    b = V^T v, partial sums of y^T v, and store v into the panel.
+//    one block per 64 x 64 tile on or below the diagonal of A22 leaves row
+//    and column partials of y = A22 v (one writer per slot) and a partial
+//    of v^T A22 v; extra blocks form the panel dots a = W^T v and
+//    b = V^T v; the grid stores v into the panel.
 //  trd_post: w = tau (y - V a - W b) + alpha v, where
 //    alpha = -tau/2 w^T v = -tau^2/2 (y^T v - 2 a^T b) needs no extra
 //    reduction pass; v goes into A; then the next panel column's update
