@@ -297,6 +297,7 @@ extern "C" int itg_mlp_infer(itg_ctx* ctx, int rows, int in, int h, int out, con
     return ITG_OK;
   }
   // other shapes: the forward's GEMMs with activations in `work`
+  if (!work) return itg_fail(ctx, ITG_EARG, "mlp_infer: this shape needs a work buffer");
   int64_t rh = (int64_t)rows * h;
   float* A1 = work;
   float* A2 = work + rh;
