@@ -343,7 +343,7 @@ def main():
                     help="short fixed workload for rocprofv3 --pmc passes (see kernel_sample)")
     ap.add_argument("--config3", action="store_true",
                     help="config-3 start: link dims 10, ramp 64..1024, cutoff 1e-12")
-    ap.add_argument("--cpu-chi", type=int, default=1024)
+    ap.add_argument("--cpu-chi", type=int, default=768)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--pc-chi", type=int, default=128,
                     help="maxdim of the converged precision check (0 = skip)")
