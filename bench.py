@@ -95,6 +95,28 @@ def run_mode(mode, N, chi, steps, warmup, krylov, config3=False, timed_kinds=())
             "kernels": {k: v for k, v in kernels.items() if v["launches"]}}
 
 
+def eigh_only(n, reps=2):
+    """Wall time of `reps` eigendecompositions of one random n x n density
+    matrix (keeping n/2, cutoff 1e-12) after a warm-up, for kernel traces
+    of the eigensolver alone."""
+    import torch
+    from itg import ops
+    A = ops.randn(n * n, 3)
+    rho = ops.empty(n * n)
+    ops.gemm("N", "T", n, n, n, 1.0 / n, A, n, A, n, 0.0, rho, n)
+    ops.eigh_trunc(ops.copy(rho), n, maxdim=n // 2, cutoff=1e-12)
+    ms = []
+    for _ in range(reps):
+        r2 = ops.copy(rho)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        ops.eigh_trunc(r2, n, maxdim=n // 2, cutoff=1e-12)
+        torch.cuda.synchronize()
+        ms.append((time.perf_counter() - t0) * 1e3)
+    log("eigh n=%d: %s ms" % (n, ["%.1f" % x for x in ms]))
+    return {"n": n, "ms": ms}
+
+
 def eigensolver_kernels(n):
     """One density-matrix eigendecomposition at the sweep's size (n = d*chi,
     maxdim n/2) with its tridiagonalisation and GEMM kernels event-timed."""
@@ -348,6 +370,8 @@ def main():
     ap.add_argument("--pc-chi", type=int, default=128,
                     help="maxdim of the converged precision check (0 = skip)")
     ap.add_argument("--pc-sweeps", type=int, default=10)
+    ap.add_argument("--eigh", type=int, default=0,
+                    help="only time two density-matrix eigendecompositions of this size (n)")
     ap.add_argument("--no-learned", action="store_true",
                     help="skip the split-BF16 + learned-preconditioner sweep")
     args = ap.parse_args()
@@ -364,6 +388,9 @@ def main():
         return
     if args.kernel_sample:
         kernel_sample(args.chi)
+        return
+    if args.eigh:
+        print(json.dumps({"eigh": eigh_only(args.eigh), "device": info}))
         return
     if args.precond_ab:
         print(json.dumps({"precond_ab": precond_ab(args.N, args.ab_chi, args.ab_sweeps,
