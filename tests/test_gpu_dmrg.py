@@ -174,7 +174,7 @@ def test_dmrg_mixed_precision_n20(itg, mode):
     E64 = dmrg(heis_mpo(N), random_mps(N, 2, 10, seed=0), sw)
     Em = dmrg(heis_mpo(N), random_mps(N, 2, 10, seed=0), sw, mode=mode)
     print(mode, E64, Em, abs(Em - E64) / N)
-    # the north-star bar, |dE|/N <= 1e-9 (measured ~6e-14 fp32, ~2e-11 bf16x3)
+    # the precision target, |dE|/N <= 1e-9 (measured ~6e-14 fp32, ~2e-11 bf16x3)
     tol = 1e-9
     assert abs(Em - E64) / N < tol
 
