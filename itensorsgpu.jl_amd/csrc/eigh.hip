@@ -351,6 +351,100 @@ __global__ __launch_bounds__(256) void trd_post_kernel(
   }
 }
 
+constexpr int TRD_TAIL = 128;   // trailing blocks this small are reduced in LDS
+
+__device__ __forceinline__ double block_sum1024(double v, double* red) {
+  v = wave_sum(v);
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double s = 0.0;
+  for (int q = 0; q < 16; ++q) s += red[q];
+  return s;
+}
+
+// The trailing block of m = n - p <= TRD_TAIL rows and columns (every
+// earlier update applied, its lower triangle current) reduced to tridiagonal
+// form in LDS by one 1024-thread block, column by column (dsytd2): y = tau
+// A22 v, w = y - tau/2 (y^T v) v, A22 -= v w^T + w v^T. d, e, tau and the
+// reflectors (v(j+1) = 1 stored) go where the per-column kernels put them.
+__global__ __launch_bounds__(1024) void trd_tail_kernel(double* __restrict__ A, int n, int p,
+                                                        double* __restrict__ d,
+                                                        double* __restrict__ e,
+                                                        double* __restrict__ tau) {
+  extern __shared__ double S[];
+  __shared__ double yp[8][TRD_TAIL];
+  __shared__ double red[16];
+  const int m = n - p, ld = m + 1;
+  double* a = S;                    // a[r + c ld], both triangles
+  double* v = S + (size_t)m * ld;   // [m]
+  double* w = v + m;                // [m]
+  const int t = threadIdx.x;
+  for (int q = t; q < m * m; q += 1024) {
+    const int r = q % m, c = q / m;
+    const int hi = r > c ? r : c, lo = r > c ? c : r;
+    a[r + c * ld] = A[(size_t)(p + hi) + (size_t)(p + lo) * n];
+  }
+  __syncthreads();
+  for (int k = 0; k + 1 < m; ++k) {
+    double s = 0.0;
+    for (int r = k + 2 + t; r < m; r += 1024) {
+      const double x = a[r + k * ld];
+      s += x * x;
+    }
+    const double xn2 = block_sum1024(s, red);
+    const double alpha = a[k + 1 + k * ld];
+    double beta = alpha, tv = 0.0, scale = 1.0;
+    if (xn2 != 0.0) {
+      beta = -copysign(sqrt(alpha * alpha + xn2), alpha);
+      tv = (beta - alpha) / beta;
+      scale = 1.0 / (alpha - beta);
+    }
+    if (t == 0) {
+      d[p + k] = a[k + k * ld];
+      e[p + k] = beta;
+      tau[p + k] = tv;
+    }
+    for (int r = t; r < m; r += 1024) {
+      const double vr = r <= k ? 0.0 : (r == k + 1 ? 1.0 : a[r + k * ld] * scale);
+      v[r] = vr;
+      if (r > k) A[(size_t)(p + r) + (size_t)(p + k) * n] = vr;
+    }
+    __syncthreads();
+    {
+      // A22 v: row r = t % TRD_TAIL, columns c = k + 1 + part (mod 8)
+      const int r = t % TRD_TAIL, part = t / TRD_TAIL;
+      double acc = 0.0;
+      if (r < m && r > k)
+        for (int c = k + 1 + part; c < m; c += 8) acc += a[r + c * ld] * v[c];
+      yp[part][r] = acc;
+    }
+    __syncthreads();
+    double yv = 0.0;
+    if (t < m) {
+      double y = 0.0;
+      if (t > k) {
+        for (int q = 0; q < 8; ++q) y += yp[q][t];
+        y *= tv;
+        yv = y * v[t];
+      }
+      w[t] = y;
+    }
+    const double ytv = block_sum1024(yv, red);
+    if (t < m && t > k) w[t] += -0.5 * tv * ytv * v[t];
+    __syncthreads();
+    for (int q = t; q < m * m; q += 1024) {
+      const int r = q % m, c = q / m;
+      if (r > k && c > k) a[r + c * ld] -= v[r] * w[c] + w[r] * v[c];
+    }
+    __syncthreads();
+  }
+  if (t == 0) {
+    d[n - 1] = a[(m - 1) + (m - 1) * ld];
+    tau[n - 1] = 0.0;
+  }
+}
+
 __global__ void trd_last_kernel(const double* __restrict__ A, int n, double* d, double* tau) {
   d[n - 1] = A[(size_t)(n - 1) * n + (n - 1)];
   tau[n - 1] = 0.0;
@@ -956,6 +1050,9 @@ int set_lds_limit(itg_ctx* ctx) {
                                    hipFuncAttributeMaxDynamicSharedMemorySize, 8192 * 12));
   ITG_HIP(ctx, hipFuncSetAttribute((const void*)dc_final_sort_kernel,
                                    hipFuncAttributeMaxDynamicSharedMemorySize, 8192 * 12));
+  ITG_HIP(ctx, hipFuncSetAttribute((const void*)trd_tail_kernel,
+                                   hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   (TRD_TAIL * (TRD_TAIL + 1) + 2 * TRD_TAIL) * 8));
   done = true;
   return ITG_OK;
 }
@@ -990,6 +1087,17 @@ static int tridiagonalize(itg_ctx* ctx, int n, double* A, double* d, double* e, 
   double* Wp = VW + (size_t)n * NB;
   double* V2 = VW + 2 * (size_t)n * NB;
   for (int p = 0; p < n; p += NB) {
+    if (n - p <= TRD_TAIL) {
+      // every earlier panel's update is applied: the rest in one block
+      if (n - p >= 2) {
+        const size_t lds = ((size_t)(n - p) * (n - p + 1) + 2 * (size_t)(n - p)) * sizeof(double);
+        trd_tail_kernel<<<1, 1024, lds, ctx->stream>>>(A, n, p, d, e, tau);
+      } else {
+        trd_last_kernel<<<1, 1, 0, ctx->stream>>>(A, n, d, tau);
+      }
+      ITG_LAUNCH_CHECK(ctx);
+      return ITG_OK;
+    }
     int nbp = std::min(NB, n - p);
     const int npn = (n + TRD_ROWS - 1) / TRD_ROWS;
     int pn_off = 0, pn_cnt = npn;
