@@ -5,17 +5,22 @@ Workload (the headline configuration): S=1/2 Heisenberg chain N=100, J=1,
 MPO bond dimension 5, two-site Davidson with 3 Krylov vectors, sweeps at
 maxdim 2048. A step is one full sweep (2(N-1) bond updates). Default start:
 a random right-canonical MPS with link dims min(2^j, 2^(N-j), 2048) (seed 1)
-and cutoff 1e-12: random states have flat spectra, so nothing is truncated
-and every bulk bond update runs at the full chi=2048 shapes (Heff on
-2048x2x2x2048, eigensolver n=4096). --config3 instead starts from
-link dims 10 with the maxdim ramp 64..1024 and cutoff 1e-12 (with which the
-bond dimension stays far below 2048 in the first sweeps). Both forms run
-from the same start: contractions in split-BF16 (3 products, FP32
-accumulation; the headline `value`) and in FP64; Davidson vectors, density
-matrices, eigensolver and energies are FP64 in both. The Heff apply is timed
-live with events on the library's stream for the roofline; the CPU baseline
-is the numpy oracle on a bounded sample (one bond update at chi=1024),
-scaled to a sweep by the chi^3 cost model.
+and cutoff 1e-12: random states have flat spectra, so every bulk bond update
+runs at the full chi=2048 shapes (Heff on 2048x2x2x2048, eigensolver
+n=4096). --config3 instead starts from link dims 10 with the maxdim ramp
+64..1024. Both forms run from the same start: contractions in split-BF16
+(3 products, FP32 accumulation; the headline `value`) and in FP64; Davidson
+vectors, environments, density matrices, eigensolver and energies are FP64
+in both.
+
+The whole run keeps to a wall budget (--budget seconds from process start):
+each form does min(--warmup, 1) untimed sweeps and then up to --steps timed
+sweeps, stopping early when its share of the budget is spent; the line's
+`steps` and `warmup` are the numbers actually run. After the two forms come,
+while time is left: the converged precision check (|dE|/N at maxdim 1024),
+the CPU baseline (the oracle's BLAS form on one bulk bond at the same
+shapes), the learned-preconditioner form and one eigendecomposition with its
+kernels timed.
 
   python bench.py [--steps K] [--warmup W] [--N 100] [--chi 2048] [--config3]
   python bench.py --micro      # Heff apply of the chi=1024 microbench only
@@ -27,6 +32,7 @@ import os
 import sys
 import time
 
+T_START = time.perf_counter()
 ROOT = os.path.dirname(os.path.abspath(__file__))
 for p in (ROOT, os.path.join(ROOT, "itensorsgpu.jl_amd")):
     if p not in sys.path:
@@ -45,13 +51,35 @@ def ramp_for(chi):
     return [m for m in (64, 128, 256, 512, 1024) if m < chi]
 
 
-def run_mode(mode, N, chi, steps, warmup, krylov, config3=False, timed_kinds=()):
-    """Sweeps at maxdim chi. Default start: a random right-canonical MPS with
-    link dims min(2^j, 2^(N-j), chi) (seed 1); its flat spectra keep every
-    bulk bond at the full chi x chi shape under cutoff 1e-12. config3: random
-    MPS with link dims 10 and the ramp 64..1024 before the chi sweeps."""
+class Budget:
+    """Wall-clock budget of the whole process, counted from its start."""
+
+    def __init__(self, seconds):
+        self.total = float(seconds)
+
+    def used(self):
+        return time.perf_counter() - T_START
+
+    def left(self):
+        return self.total - self.used()
+
+
+class OutOfTime(Exception):
+    pass
+
+
+def run_mode(mode, N, chi, steps, warmup, krylov, budget, share, config3=False,
+             timed_kinds=(), factorize_kinds=()):
+    """`warmup` untimed sweeps, then up to `steps` timed sweeps at maxdim chi,
+    stopping early (after at least one) when the next sweep would end past
+    `share` seconds of timed work or past the process budget. Default start:
+    a random right-canonical MPS with link dims min(2^j, 2^(N-j), chi)
+    (seed 1); its flat spectra keep every bulk bond at the full chi x chi
+    shape under cutoff 1e-12. config3: random MPS with link dims 10 and the
+    ramp 64..1024 before the chi sweeps. The timed region is bracketed by
+    device synchronisation; every sweep's wall time is recorded."""
     from itg import _lib
-    from itg.mps import MPO, PhaseTimer, Sweeps, dmrg, heisenberg_opsum, random_mps
+    from itg.mps import MPO, PhaseTimer, ProjMPO, Sweeps, dmrg, heisenberg_opsum, random_mps
     import torch
     H = MPO.from_opsum(heisenberg_opsum([(j, j + 1) for j in range(N - 1)]), N)
     t0 = time.perf_counter()
@@ -66,33 +94,54 @@ def run_mode(mode, N, chi, steps, warmup, krylov, config3=False, timed_kinds=())
         psi = random_mps(N, 2, chi, seed=1)
         log("[%s] random MPS chi=%d ready (%.1f s)" % (mode, chi, time.perf_counter() - t0))
         ph = None
+    wrec = []
     for _ in range(warmup):
         dmrg(H, psi, Sweeps(1, maxdim=[chi], cutoff=[cutoff]), mode=mode, krylovdim=krylov,
-             ph=ph, sweep_callback=lambda r: log("[%s] warmup" % mode, r))
+             ph=ph, record=wrec, sweep_callback=lambda r: log("[%s] warmup" % mode, r))
         ph = dmrg.last_projmpo
     if ph is None:
-        from itg.mps import ProjMPO
         ph = ProjMPO(H, mode)
         ph.build_right(psi)
     setup_s = time.perf_counter() - t0
     log("[%s] set-up done (%.1f s)" % (mode, setup_s))
-    timer = PhaseTimer()
+    # the middle bond (full chi x chi shape) has its eigensolver kernels
+    # event-timed inside the first timed sweep
+    timer = PhaseTimer(factorize_bond=N // 2 if factorize_kinds else -1,
+                       factorize_kinds=factorize_kinds)
+    timer.base_kinds = tuple(timed_kinds)
     rec = []
     torch.cuda.synchronize()
     _lib.ctx().sync()
     _lib.ctx().timing(timed_kinds)
     t1 = time.perf_counter()
-    E = dmrg(H, psi, Sweeps(steps, maxdim=[chi], cutoff=[cutoff]), mode=mode, krylovdim=krylov,
+    done = 0
+    while done < steps:
+        dmrg(H, psi, Sweeps(1, maxdim=[chi], cutoff=[cutoff]), mode=mode, krylovdim=krylov,
              ph=ph, timer=timer, record=rec,
              sweep_callback=lambda r: log("[%s] timed" % mode, r))
+        ph = dmrg.last_projmpo
+        timer.factorize_kinds = ()
+        done += 1
+        el = time.perf_counter() - t1
+        per = el / done
+        if el + per > share or budget.left() < per + 30.0:
+            break
     torch.cuda.synchronize()
     _lib.ctx().sync()
     t2 = time.perf_counter()
-    kernels = {_lib.KERNEL_NAMES[k]: _lib.ctx().timing_report(k) for k in timed_kinds}
+    kinds = tuple(timed_kinds) + tuple(factorize_kinds)
+    kernels = {_lib.KERNEL_NAMES[k]: _lib.ctx().timing_report(k) for k in kinds}
     _lib.ctx().timing(())
-    return {"s_per_sweep": (t2 - t1) / steps, "energy": E, "maxlinkdim": psi.maxlinkdim(),
+    E = rec[-1]["energy"]
+    return {"s_per_sweep": (t2 - t1) / done, "steps": done, "warmup": warmup,
+            "timed_s": t2 - t1, "energy": E, "maxlinkdim": psi.maxlinkdim(),
             "setup_s": setup_s, "phases": timer.report(), "sweeps": rec,
-            "kernels": {k: v for k, v in kernels.items() if v["launches"]}}
+            "warmup_sweeps": [{k: r[k] for k in ("energy", "seconds", "maxerr", "matvecs")}
+                              for r in wrec],
+            "kernels": {k: v for k, v in kernels.items() if v["launches"]},
+            "kernel_sets": {"sweep": [_lib.KERNEL_NAMES[k] for k in timed_kinds],
+                            "bond_%d_factorize" % (N // 2):
+                                [_lib.KERNEL_NAMES[k] for k in factorize_kinds]}}
 
 
 def eigh_only(n, reps=2):
@@ -140,22 +189,37 @@ def eigensolver_kernels(n):
     return out
 
 
-def precision_check(N, chi, nsweep, krylov, modes):
+def precision_check(N, chi, nsweep, krylov, modes, budget, reserve):
     """Every arithmetic form from the same start (random MPS, link dims 10,
     seed 2) through nsweep sweeps with maxdim ramping to chi, cutoff 1e-12:
-    converged FP64 energies <psi|H|psi>, compared per site."""
+    converged FP64 energies <psi|H|psi>, compared per site. Stops (and
+    reports which forms finished) when the budget would drop below
+    `reserve` seconds."""
     from itg.mps import MPO, Sweeps, dmrg, heisenberg_opsum, random_mps
     H = MPO.from_opsum(heisenberg_opsum([(j, j + 1) for j in range(N - 1)]), N)
     ramp = [m for m in (16, 32, 64, 128, 256, 512) if m < chi] + [chi]
-    out = {}
+    out, recs = {}, {}
+
+    def guard(r, m):
+        log("[%s] precision check" % m, r)
+        if budget.left() < reserve:
+            raise OutOfTime()
+
     for m in modes:
         t0 = time.perf_counter()
         psi = random_mps(N, 2, 10, seed=2)
-        out[m] = dmrg(H, psi, Sweeps(nsweep, maxdim=ramp, cutoff=[1e-12]), mode=m,
-                      krylovdim=krylov)
+        rec = []
+        try:
+            out[m] = dmrg(H, psi, Sweeps(nsweep, maxdim=ramp, cutoff=[1e-12]), mode=m,
+                          krylovdim=krylov, record=rec, sweep_callback=lambda r: guard(r, m))
+        except OutOfTime:
+            log("[%s] precision check stopped after %d sweeps: out of time" % (m, len(rec)))
+            return None, {"stopped": m, "sweeps_done": len(rec)}
+        recs[m] = [{"energy": r["energy"], "maxlinkdim": r["maxlinkdim"],
+                    "maxerr": r["maxerr"], "seconds": r["seconds"]} for r in rec]
         log("[%s] precision check: E = %.15f after %d sweeps (%.1f s)"
             % (m, out[m], nsweep, time.perf_counter() - t0))
-    return out
+    return out, recs
 
 
 def kernel_sample(chi, reps=3):
@@ -280,37 +344,46 @@ def sweep_cost_units(N, d, chi, w):
 
 
 def cpu_baseline(chi, w=5, d=2, krylov=3, seed=0):
-    """The oracle (numpy FP64, BLAS/LAPACK) on one bond update at bond
-    dimension chi: krylov Heff applies, the density matrix and its
-    eigendecomposition, one environment update. Returns seconds."""
+    """The oracle's BLAS form (numpy FP64, OpenBLAS/LAPACK) on one bulk bond
+    update at bond dimension chi, the same shapes as the GPU sweep's bulk
+    bonds: `krylov` Heff applies, the density matrix and its
+    eigendecomposition (LAPACK syevd), the new site tensors and one
+    environment update. Returns seconds."""
     import numpy as np
     from oracle import dmrg as odmrg
-    from oracle import linalg as olin
     rng = np.random.default_rng(seed)
     L = rng.standard_normal((chi, w, chi))
     R = rng.standard_normal((chi, w, chi))
     W = rng.standard_normal((w, w, d, d))
     psi = rng.standard_normal((chi, d, d, chi))
-    A = rng.standard_normal((chi, d, chi)) / chi
     t0 = time.perf_counter()
     x = psi
     for i in range(krylov):
-        x = odmrg.heff_apply(L, W, W, R, x)
+        x = odmrg.heff_apply_blas(L, W, W, R, x)
         x = x / np.linalg.norm(x)
         log("cpu baseline: Heff apply %d done (%.1f s)" % (i + 1, time.perf_counter() - t0))
     P = np.reshape(x, (chi * d, d * chi), order="F")
-    olin.eigh_trunc(P @ P.T, maxdim=chi, cutoff=1e-12)
+    ev, U = np.linalg.eigh(P @ P.T)
+    U = U[:, ::-1][:, :chi]
+    A = np.reshape(U, (chi, d, chi), order="F")
+    B = U.T @ P
     log("cpu baseline: eigh n=%d done (%.1f s)" % (chi * d, time.perf_counter() - t0))
-    odmrg.update_left(L, A, W)
+    odmrg.update_left_blas(L, A, W)
     return time.perf_counter() - t0
 
 
 def blas_threads():
+    """Threads the host BLAS runs with (threadpoolctl), else the CPUs this
+    process may use."""
     try:
         from threadpoolctl import threadpool_info
-        return max([i.get("num_threads", 1) for i in threadpool_info()] or [1])
+        n = max([i.get("num_threads", 0) for i in threadpool_info()
+                 if i.get("user_api") == "blas"] or [0])
+        if n:
+            return n
     except Exception:  # noqa: BLE001
-        return 1
+        pass
+    return len(os.sched_getaffinity(0))
 
 
 def micro(reps=10, chi=1024):
@@ -347,11 +420,26 @@ def micro(reps=10, chi=1024):
     return out
 
 
+def roofline_from(kd, bound, peak, unit, launch_set):
+    """roofline object of one kernel class from its event-timed launches."""
+    n = max(kd["launches"], 1)
+    if bound == "mfma":
+        achieved = kd["flops"] / kd["ms"] / 1e9          # TFLOP/s
+    else:
+        achieved = kd["bytes"] / kd["ms"] / 1e6          # GB/s
+    return {"bound": bound, "achieved": achieved, "peak": peak, "unit": unit,
+            "frac": achieved / peak, "traffic": None, "launches": kd["launches"],
+            "avg_ms": kd["ms"] / n, "flops_per_launch": kd["flops"] / n,
+            "bytes_per_launch": kd["bytes"] / n, "launch_set": launch_set}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=1)
-    ap.add_argument("--warmup", type=int, default=0)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--budget", type=float, default=450.0,
+                    help="wall seconds for the whole run, counted from process start")
     ap.add_argument("--N", type=int, default=100)
     ap.add_argument("--chi", type=int, default=2048)
     ap.add_argument("--krylov", type=int, default=3)
@@ -365,9 +453,10 @@ def main():
                     help="short fixed workload for rocprofv3 --pmc passes (see kernel_sample)")
     ap.add_argument("--config3", action="store_true",
                     help="config-3 start: link dims 10, ramp 64..1024, cutoff 1e-12")
-    ap.add_argument("--cpu-chi", type=int, default=768)
+    ap.add_argument("--cpu-chi", type=int, default=0,
+                    help="bond dimension of the CPU baseline's sample (0 = --chi)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--pc-chi", type=int, default=128,
+    ap.add_argument("--pc-chi", type=int, default=1024,
                     help="maxdim of the converged precision check (0 = skip)")
     ap.add_argument("--pc-sweeps", type=int, default=10)
     ap.add_argument("--eigh", type=int, default=0,
@@ -396,69 +485,123 @@ def main():
         print(json.dumps({"precond_ab": precond_ab(args.N, args.ab_chi, args.ab_sweeps,
                                                    args.krylov), "device": info}))
         return
+    budget = Budget(args.budget)
     modes = args.modes.split(",")
-    from itg import _lib as L_
+    L_ = _lib
     res = {}
     head = modes[0]
-    # the headline form's timed sweeps event-time every split-GEMM and W-pass
-    # launch (about 1.2k per sweep); they are the first such launches of the
-    # process, so a kernel trace of this command can be cut to the same set
+    warm = min(args.warmup, 1)
+    notes = []
+    if warm < args.warmup:
+        notes.append("warmup capped at 1 sweep per form (asked %d) to keep the run within "
+                     "the %.0f s budget" % (args.warmup, args.budget))
+    # the headline form event-times every split-GEMM and W-pass launch of its
+    # timed sweeps, plus the eigensolver kernels of the middle bond's
+    # factorisation in the first timed sweep
     timed = (L_.K_GEMM_SPLIT, L_.K_WPASS) if head == "bf16x3" else (L_.K_GETT, L_.K_WPASS)
+    fk = (L_.K_TRD_SYMV, L_.K_TRD_POST)
+    shares = {head: 0.30 * args.budget}
+    for m in modes[1:]:
+        shares[m] = 0.12 * args.budget
     for m in modes:
-        res[m] = run_mode(m, args.N, args.chi, args.steps, args.warmup, args.krylov,
-                          args.config3, timed_kinds=timed if m == head else ())
+        res[m] = run_mode(m, args.N, args.chi, args.steps, warm, args.krylov, budget,
+                          shares[m], args.config3, timed_kinds=timed if m == head else (),
+                          factorize_kinds=fk if m == head else ())
         log(m, {k: v for k, v in res[m].items() if k != "sweeps"})
+        if res[m]["steps"] < args.steps:
+            notes.append("%s: %d of %d timed sweeps fit its share of the budget"
+                         % (m, res[m]["steps"], args.steps))
     r = res[head]
     heff = r["phases"].get("davidson") or r["phases"].get("heff") or \
         {"ms": float("nan"), "flops": 0.0, "calls": 0}
-    pc = None
+    # converged precision check; reserve time for the CPU baseline
+    pc, pc_rec = None, None
     if len(modes) > 1 and args.pc_chi > 0:
-        pc = precision_check(args.N, args.pc_chi, args.pc_sweeps, args.krylov, modes)
+        if budget.left() > 150:
+            pc, pc_rec = precision_check(args.N, args.pc_chi, args.pc_sweeps, args.krylov,
+                                         modes, budget, reserve=60.0)
+        else:
+            notes.append("precision check skipped: %.0f s left" % budget.left())
+    cpu = None
+    if not args.no_cpu_baseline:
+        cchi = args.cpu_chi or args.chi
+        if budget.left() > 45:
+            t_bond = cpu_baseline(cchi)
+            units = sweep_cost_units(args.N, 2, args.chi, 5) * (args.chi / cchi) ** 3
+            cpu = {"value": t_bond * units, "unit": "s/sweep", "cores": blas_threads(),
+                   "kind": "port",
+                   "sample": "oracle BLAS form (numpy FP64, OpenBLAS/LAPACK) on one bulk bond "
+                             "update at chi=%d (w=5, d=2): %d Heff applies + density matrix + "
+                             "eigh n=%d + site tensors + one environment update = %.2f s, times "
+                             "%.1f bond equivalents per sweep (the sweep's Heff FLOPs in units "
+                             "of one chi=%d bond)"
+                             % (cchi, args.krylov, 2 * cchi, t_bond, units, args.chi)}
+            log("cpu baseline", cpu)
+        else:
+            notes.append("CPU baseline skipped: %.0f s left" % budget.left())
     learned = None
     if head == "bf16x3" and not args.no_learned and not args.config3:
-        learned = learned_form(args.N, args.chi, args.krylov)
-        log("learned form", learned)
+        est = 1.3 * r["s_per_sweep"] + 30.0
+        if budget.left() > est + 20:
+            learned = learned_form(args.N, args.chi, args.krylov)
+            log("learned form", {k: v for k, v in learned.items() if k != "phases"})
+        else:
+            notes.append("learned form skipped: %.0f s left, ~%.0f s needed"
+                         % (budget.left(), est))
+    eig = None
+    if budget.left() > 15:
+        eig = eigensolver_kernels(2 * args.chi)
+        log("eigensolver", eig)
     kern = r.get("kernels", {})
-    mfma_kern = {k: v for k, v in kern.items()
-                 if k != L_.KERNEL_NAMES[L_.K_WPASS] and v["launches"]}
-    eig = eigensolver_kernels(2 * args.chi)
-    log("eigensolver", eig)
-    if mfma_kern:
-        kname, kd = max(mfma_kern.items(), key=lambda kv: kv[1]["ms"])
-        kms = kd["ms"] / max(kd["launches"], 1)
-        roof = {"bound": "mfma", "kernel": kname, "achieved": kd["flops"] / kd["ms"] / 1e9,
-                "peak": PEAK[head], "unit": "TFLOP/s", "traffic": None,
-                "launches": kd["launches"], "avg_ms": kms,
-                "flops_per_launch": kd["flops"] / max(kd["launches"], 1),
-                "launch_set": "every launch of the timed sweeps (the process's first %d)"
-                              % kd["launches"]}
-        roof["frac"] = roof["achieved"] / roof["peak"]
-        # HBM bytes per launch from the committed FETCH_SIZE/WRITE_SIZE passes
-        # (bulk-bond shapes; see the file for the commands and calibration)
+    # roofline of the Heff's split GEMM (every launch of the timed sweeps)
+    roof_heff = None
+    kname = L_.KERNEL_NAMES[L_.K_GEMM_SPLIT if head == "bf16x3" else L_.K_GETT]
+    if kname in kern:
+        roof_heff = roofline_from(kern[kname], "mfma", PEAK[head], "TFLOP/s",
+                                  "every launch of the timed sweeps")
+        roof_heff["kernel"] = kname
         tpath = os.path.join(ROOT, "profiles", "gemm_split_kernel_traffic_chi2048.json")
         if kname.startswith("gemm_split_kernel") and os.path.exists(tpath):
             tr = json.load(open(tpath))
-            roof["traffic"] = tr["bytes_per_launch"]
-            roof["traffic_shape"] = tr["shape"]
-            roof["traffic_algorithmic"] = tr["algorithmic_bytes_per_launch"]
-    else:
-        achieved = heff["flops"] / (heff["ms"] / 1e3) / 1e12 if heff["ms"] > 0 else float("nan")
-        roof = {"bound": "mfma", "kernel": "two-site Heff apply (C1 + W pass + C4)",
-                "achieved": achieved, "peak": PEAK[head], "unit": "TFLOP/s",
-                "frac": achieved / PEAK[head], "traffic": None, "launches": heff["calls"],
-                "avg_ms": heff["ms"] / max(heff["calls"], 1)}
-    symv = eig["kernels"][L_.KERNEL_NAMES[L_.K_TRD_SYMV]]
-    roof_eig = {"bound": "hbm", "kernel": L_.KERNEL_NAMES[L_.K_TRD_SYMV],
-                "achieved": symv["bytes"] / symv["ms"] / 1e6, "peak": HBM_PEAK, "unit": "GB/s",
-                "launches": symv["launches"], "avg_ms": symv["ms"] / max(symv["launches"], 1)}
-    roof_eig["frac"] = roof_eig["achieved"] / HBM_PEAK
+            roof_heff["traffic"] = tr["bytes_per_launch"]
+            roof_heff["traffic_shape"] = tr["shape"]
+            roof_heff["traffic_algorithmic"] = tr["algorithmic_bytes_per_launch"]
+    # roofline of the tridiagonalisation's symmetric product (the kernel with
+    # the most time per sweep), timed inside the first timed sweep at the
+    # middle bond (n = 2 chi)
+    roof_symv = None
+    sname = L_.KERNEL_NAMES[L_.K_TRD_SYMV]
+    if sname in kern:
+        roof_symv = roofline_from(kern[sname], "hbm", HBM_PEAK, "GB/s",
+                                  "every launch of the factorisation of bond %d in the first "
+                                  "timed sweep (n=%d)" % (args.N // 2, 2 * args.chi))
+        roof_symv["kernel"] = sname
+        tpath = os.path.join(ROOT, "profiles", "trd_symv_tiles_kernel_traffic_n4096.json")
+        if os.path.exists(tpath):
+            tr = json.load(open(tpath))
+            roof_symv["traffic"] = tr["bytes_per_launch"]
+            roof_symv["traffic_algorithmic"] = tr["algorithmic_bytes_per_launch"]
+    # which of the two holds more of the sweep: the symv share of the
+    # factorisation phase (from the timed bond) against the GEMM's total
+    dominant = roof_heff
+    per_sweep = {}
+    if roof_heff is not None:
+        per_sweep[roof_heff["kernel"]] = kern[roof_heff["kernel"]]["ms"] / r["steps"]
+    if roof_symv is not None and "factorize_timed_bond" in r["phases"]:
+        # the symv's share of the timed bond's factorisation, times the
+        # sweep's factorisation time (bulk bonds share the timed bond's shape)
+        fact = r["phases"]["factorize"]["ms"] / r["steps"]
+        share = kern[sname]["ms"] / max(r["phases"]["factorize_timed_bond"]["ms"], 1e-9)
+        per_sweep[sname] = share * fact
+        if roof_heff is None or per_sweep[sname] > per_sweep[roof_heff["kernel"]]:
+            dominant = roof_symv
     line = {
         "metric": METRIC,
         "value": r["s_per_sweep"],
         "unit": "s/sweep",
         "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
+        "steps": r["steps"],
+        "warmup": r["warmup"],
         "ms_per_step": r["s_per_sweep"] * 1e3,
         "higher_is_better": False,
         "scaling": "weak",
@@ -471,16 +614,20 @@ def main():
                                 "then sweeps at maxdim %d" % (ramp_for(args.chi), args.chi)
                                 if args.config3 else
                                 "random right-canonical MPS at link dims min(2^j, 2^(N-j), %d) "
-                                "seed 1, cutoff 1e-12 (flat random spectra: bulk bonds stay at %d)" % (args.chi, args.chi)) +
+                                "seed 1, cutoff 1e-12 (flat random spectra: bulk bonds stay at %d)"
+                                % (args.chi, args.chi)) +
                                "; one step = one sweep at maxdim %d" % args.chi,
                    "N": args.N, "chi": args.chi, "mpo_bond_dim": 5, "modes": modes},
-        "roofline": roof,
-        "roofline_eigensolver": roof_eig,
+        "roofline": dominant,
+        "roofline_heff_gemm": roof_heff,
+        "roofline_tridiagonalisation": roof_symv,
+        "kernel_ms_per_sweep": per_sweep,
         "davidson": {"bonds": heff["calls"], "avg_ms": heff["ms"] / max(heff["calls"], 1),
                      "heff_tflops": heff["flops"] / max(heff["ms"], 1e-9) / 1e9},
         "eigensolver": eig,
         "results": {m: {k: v for k, v in res[m].items() if k != "sweeps"} for m in modes},
-        "kernels": kern,
+        "budget_s": args.budget,
+        "notes": notes,
     }
     if len(modes) > 1:
         e = [res[m]["energy"] for m in modes]
@@ -494,25 +641,17 @@ def main():
                 "workload": "same chain, random MPS link dims 10 seed 2, %d sweeps, maxdim ramp "
                             "to %d, cutoff 1e-12; FP64 energies of the converged states"
                             % (args.pc_sweeps, args.pc_chi),
-                "energy": pc}
+                "energy": pc, "sweeps": pc_rec}
+        elif pc_rec is not None:
+            line["precision_check"] = {"incomplete": pc_rec}
     if learned is not None:
         learned["energy_plain_sweep"] = r["energy"]
         if len(modes) > 1:
             learned["speedup_vs_" + modes[1]] = res[modes[1]]["s_per_sweep"] / learned["s_per_sweep"]
         line["learned_form"] = learned
-    if not args.no_cpu_baseline:
-        cchi = min(args.cpu_chi, args.chi)
-        t_bond = cpu_baseline(cchi)
-        # bond-cost units of a sweep relative to one chi=cchi bond (Heff and
-        # eigh both scale as chi^3 at fixed w, d)
-        units = sweep_cost_units(args.N, 2, args.chi, 5) * (args.chi / cchi) ** 3
-        line["cpu_baseline"] = {
-            "value": t_bond * units, "unit": "s/sweep", "cores": blas_threads(),
-            "kind": "port",
-            "sample": "oracle (numpy FP64, OpenBLAS/LAPACK) one bond update at chi=%d: %d Heff "
-                      "applies + density-matrix eigh (n=%d) + one environment update = %.2f s, "
-                      "times %.1f such bonds per sweep (chi^3 cost model)"
-                      % (cchi, args.krylov, 2 * cchi, t_bond, units)}
+    if cpu is not None:
+        line["cpu_baseline"] = cpu
+    line["wall_s"] = budget.used()
     print(json.dumps(line))
 
 

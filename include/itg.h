@@ -83,6 +83,9 @@ enum {
   ITG_K_COUNT = 5
 };
 int itg_timing(itg_ctx* ctx, int mask);
+/* Change the set of timed kernel classes without clearing what was recorded
+ * (to time some classes over a sub-range of a run). */
+int itg_timing_set(itg_ctx* ctx, int mask);
 int itg_timing_report(itg_ctx* ctx, int kind, int64_t* launches, double* ms,
                       double* flops, double* bytes);
 
@@ -211,7 +214,8 @@ int itg_qr(itg_ctx* ctx, int dtype, int64_t m, int64_t n, const void* A,
  * dmrg, test/dmrg.jl:27): contraction order L.psi -> .W1 -> .W2 -> .R, with
  * W1.W2 fused into one (wl d d) x (d d wr) operator applied in a single HBM
  * pass; the two large steps run on MFMA in `mode`, the W pass in FP64.
- * `work` must hold itg_heff2_work_elems() elements of dtype. */
+ * `work` must hold itg_heff2_work_elems() 4-byte units (the same count serves
+ * FP64 and FP32 applies). */
 int64_t itg_heff2_work_elems(int64_t chiL, int64_t wl, int64_t wm, int64_t wr,
                              int64_t d, int64_t chiR);
 int itg_heff2_apply(itg_ctx* ctx, int dtype, int mode, int64_t chiL, int64_t wl,
@@ -242,7 +246,8 @@ int itg_heff2_apply_prepared(itg_ctx* ctx, int dtype, int mode, int64_t chiL,
  * on the host, ITensors' stopping rule on the residual norm and errgoal.
  * psi (FP64, chiL x d x d x chiR) is the start vector and receives the
  * normalised Ritz vector; diag (the operator's diagonal) selects the
- * diagonal preconditioner, null none. work: itg_heff2_work_elems(); kwork:
+ * diagonal preconditioner, null none. work: itg_heff2_work_elems() 4-byte
+ * units; kwork:
  * itg_davidson2_work_elems() doubles. In ITG_MODE_BF16X3 the Heff apply is
  * captured once per (shape, buffers) into a HIP graph and replayed. */
 int64_t itg_davidson2_work_elems(int64_t chiL, int64_t d, int64_t chiR, int maxiter);

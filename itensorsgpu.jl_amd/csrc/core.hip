@@ -144,6 +144,12 @@ extern "C" int itg_timing(itg_ctx* ctx, int mask) {
   return ITG_OK;
 }
 
+extern "C" int itg_timing_set(itg_ctx* ctx, int mask) {
+  ITG_CHECK_CTX(ctx);
+  ctx->timing = mask & ((1 << ITG_K_COUNT) - 1);
+  return ITG_OK;
+}
+
 extern "C" int itg_timing_report(itg_ctx* ctx, int kind, int64_t* launches, double* ms,
                                  double* flops, double* bytes) {
   ITG_CHECK_CTX(ctx);

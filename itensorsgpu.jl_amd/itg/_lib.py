@@ -56,6 +56,7 @@ _SIGS = {
     "itg_set_stream": (C.c_int, [_ctxp, _vp]),
     "itg_device_info": (C.c_int, [_ctxp, C.POINTER(C.c_int), C.POINTER(C.c_int), _dp]),
     "itg_timing": (C.c_int, [_ctxp, C.c_int]),
+    "itg_timing_set": (C.c_int, [_ctxp, C.c_int]),
     "itg_timing_report": (C.c_int, [_ctxp, C.c_int, _i64p, _dp, _dp, _dp]),
     "itg_malloc": (C.c_int, [_ctxp, C.c_size_t, C.POINTER(_vp)]),
     "itg_free": (C.c_int, [_ctxp, _vp]),
@@ -194,6 +195,13 @@ class Context:
         for k in kinds:
             mask |= 1 << int(k)
         self.check(lib().itg_timing(self.ptr, mask))
+
+    def timing_set(self, kinds=()):
+        """Change the timed kernel classes, keeping what was recorded."""
+        mask = 0
+        for k in kinds:
+            mask |= 1 << int(k)
+        self.check(lib().itg_timing_set(self.ptr, mask))
 
     def timing_report(self, kind):
         """{launches, ms, flops, bytes} of one kernel class (K_* below)."""

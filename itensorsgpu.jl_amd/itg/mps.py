@@ -237,17 +237,28 @@ def right_orthogonalize(psi):
 # ------------------------------------------------------------------ timing
 class PhaseTimer:
     """Accumulates device time per phase with events recorded on the stream
-    libitg runs on, plus the algorithmic FLOPs of each timed call."""
+    libitg runs on, plus the algorithmic FLOPs of each timed call.
+    `factorize_kinds` (libitg kernel classes) are additionally event-timed
+    during the factorisation of bond `factorize_bond` when the sweep passes
+    it left to right (the library's own per-launch timing, added to whatever
+    classes are being timed already)."""
 
-    def __init__(self):
+    def __init__(self, factorize_bond=-1, factorize_kinds=()):
         self.pairs = {}
         self.flops = {}
         self.calls = {}
+        self.factorize_bond = factorize_bond
+        self.factorize_kinds = tuple(factorize_kinds)
+        self.base_kinds = ()
 
     def start(self):
         import torch
         ev = torch.cuda.Event(enable_timing=True)
         ev.record()
+        return ev
+
+    def start_at(self, ev):
+        """An existing start event, for a second phase ending at the next stop."""
         return ev
 
     def stop(self, name, ev0, flops=0.0):
@@ -307,7 +318,23 @@ class ProjMPO:
         self.L = [None] * (N + 1)
         self.R = [None] * (N + 1)
         self._prep = None        # (bond, mode, dims, buffer) of itg_heff2_prepare
-        self._bufs = {}          # Heff and Krylov work buffers per shape (reused)
+        # grow-only buffers shared by every bond: one each for the prepared
+        # operator, the Heff work and the Krylov basis, sized for the largest
+        # shape seen so far (bonds of a truncating sweep all differ in shape,
+        # so per-shape buffers would pile up)
+        self._bufs = {}
+
+    def buffer(self, name, nelem, dtype):
+        """Grow-only buffer `name` of at least nelem elements of dtype."""
+        b = self._bufs.get(name)
+        if b is None or b.numel() < nelem or str(b.dtype).split(".")[-1] != dtype:
+            self._bufs.pop(name, None)
+            b = ops.empty(nelem, dtype)
+            self._bufs[name] = b
+        return b
+
+    def heff_work(self, dims):
+        return self.buffer("work", _lib.lib().itg_heff2_work_elems(*dims), "float32")
 
     def build_right(self, psi, upto=2):
         N = len(psi)
@@ -327,17 +354,16 @@ class ProjMPO:
         wr = self.H.dims[b + 1][1]
         chiL, d, _, chiR = xdims
         dims = (chiL, wl, wm, wr, d, chiR)
-        buf = None
-        if self._prep is not None and self._prep[2] == dims:
-            buf = self._prep[3]
+        buf = self.buffer("prep", _lib.lib().itg_heff2_prep_elems(*dims), "float32")
         buf = ops.heff2_prepare(L, self.H.data[b], self.H.data[b + 1], R, *dims, mode=self.mode,
                                 prep=buf)
         self._prep = (b, self.mode, dims, buf, L, R)
 
     def eigsolve(self, b, phi, xdims, maxiter=2, errgoal=1e-14, diag=None, stats=None):
         """Native device Davidson (itg_davidson2) on the prepared bond b;
-        phi is overwritten with the Ritz vector. The work buffers are kept
-        per shape so that split-BF16 applies replay their captured graphs."""
+        phi is overwritten with the Ritz vector. The work buffers are the
+        ProjMPO's grow-only ones, so bonds of one shape replay the split-BF16
+        graphs captured for it."""
         L, Ld = self.L[b]
         R, Rd = self.R[b + 2]
         p = self._prep
@@ -345,11 +371,10 @@ class ProjMPO:
             self.prepare(b, xdims)
             p = self._prep
         dims = p[2]
-        key = (dims, maxiter)
-        if key not in self._bufs:
-            self._bufs[key] = (ops.heff2_work(*dims, "float64"),
-                               ops.davidson2_work(dims[0], dims[4], dims[5], maxiter))
-        work, kwork = self._bufs[key]
+        work = self.heff_work(dims)
+        kwork = self.buffer("kwork", _lib.lib().itg_davidson2_work_elems(dims[0], dims[4],
+                                                                           dims[5], maxiter),
+                            "float64")
         ev = self.timer.start() if self.timer is not None else None
         lam, nmv = ops.davidson2(L, R, p[3], phi, *dims, maxiter=maxiter, errgoal=errgoal,
                                  diag=diag, mode=self.mode, work=work, kwork=kwork)
@@ -483,24 +508,67 @@ def diag_preconditioner(dg):
 
 
 # --------------------------------------------------------------- factorize
-def replacebond(psi, b, phi, pdims, ortho, maxdim, mindim, cutoff):
-    """replacebond!(psi, b, phi; ortho, normalize=true) via factorize."""
+def noiseterm(PH, b, phi, pdims, ortho, mode="native"):
+    """The density-matrix perturbation of ITensors' two-site dmrg with
+    noise (`drho = noise * noiseterm(PH, phi, ortho)`, used through
+    `noise!` at examples/dmrg.jl:24 and test/dmrg.jl:24): the half of the
+    projected operator on the side being kept, applied to phi with its MPO
+    link left open, then traced over everything but that side's indices.
+      left:  nt(a', s1', w, s2, b) = L_b(a', wl, a) W_b(wl, w, s1', s1) phi(a, s1, s2, b)
+             drho[(a s1), (a' s1')] = sum_{w s2 b} nt nt       (m x m, m = chiL d)
+      right: nt(a, s1, w, s2', b') = phi(a, s1, s2, b) R(b', wr, b) W_{b+1}(w, wr, s2', s2)
+             drho[(s2 b), (s2' b')] = sum_{a s1 w} nt nt       (n x n, n = d chiR)
+    The large contraction and the Gram product run in `mode` (the term is
+    scaled by the noise, 1e-5 and below); the result is exactly symmetric."""
+    chiL, d1, d2, chiR = pdims
+    dt = str(phi.dtype).split(".")[-1]
+    if ortho == "left":
+        L, Ld = PH.L[b]
+        W, Wd = PH.H.data[b], PH.H.dims[b]
+        T, dT = ops.contract(L, Ld, [1, 2, 3], phi, pdims, [3, 4, 5, 6], [1, 2, 4, 5, 6],
+                             mode=mode)
+        nt, dn = ops.contract(T, dT, [1, 2, 4, 5, 6], W, Wd, [2, 7, 8, 4], [1, 8, 7, 5, 6])
+        m, K = chiL * d1, Wd[1] * d2 * chiR
+        drho = ops.empty(m * m, dt)
+        ops.gemm("N", "T", m, m, K, 1.0, nt, m, nt, m, 0.0, drho, m, mode=mode)
+    else:
+        R, Rd = PH.R[b + 2]
+        W, Wd = PH.H.data[b + 1], PH.H.dims[b + 1]
+        T, dT = ops.contract(phi, pdims, [1, 2, 3, 4], R, Rd, [5, 6, 4], [1, 2, 3, 5, 6],
+                             mode=mode)
+        nt, dn = ops.contract(T, dT, [1, 2, 3, 5, 6], W, Wd, [7, 6, 8, 3], [1, 2, 7, 8, 5])
+        n, K = d2 * chiR, chiL * d1 * Wd[0]
+        drho = ops.empty(n * n, dt)
+        ops.gemm("T", "N", n, n, K, 1.0, nt, K, nt, K, 0.0, drho, n, mode=mode)
+    return drho
+
+
+def replacebond(psi, b, phi, pdims, ortho, maxdim, mindim, cutoff, drho=None, noise=0.0):
+    """replacebond!(psi, b, phi; ortho, normalize=true) via factorize. With
+    a density-matrix perturbation `drho` (scaled by `noise`) the eigen route
+    is taken on rho + noise*drho, as factorize does for
+    eigen_perturbation."""
     l, d1, d2, r = pdims
     m, n = l * d1, d2 * r
-    use_eigen = cutoff is not None and cutoff >= 1e-12
+    use_eigen = (cutoff is not None and cutoff >= 1e-12) or drho is not None
     dt = str(phi.dtype).split(".")[-1]
     if use_eigen:
         if ortho == "left":
             rho = ops.empty(m * m, dt)
             ops.gemm("N", "T", m, m, n, 1.0, phi, m, phi, m, 0.0, rho, m)
+            if drho is not None:
+                ops.axpby(noise, drho, 1.0, rho)
             w, U, k, terr, _ = ops.eigh_trunc(rho, m, maxdim=maxdim, cutoff=cutoff)
-            A = U[: m * k]
+            # own copy of the kept columns: the n x n eigenvector buffer is freed
+            A = U[: m * k].clone()
             B = ops.empty(k * n, dt)
             ops.gemm("T", "N", k, n, m, 1.0, U, m, phi, m, 0.0, B, k)
             ops.scal(B, 1.0 / ops.nrm2(B))
         else:
             rho = ops.empty(n * n, dt)
             ops.gemm("T", "N", n, n, m, 1.0, phi, m, phi, m, 0.0, rho, n)
+            if drho is not None:
+                ops.axpby(noise, drho, 1.0, rho)
             w, U, k, terr, _ = ops.eigh_trunc(rho, n, maxdim=maxdim, cutoff=cutoff)
             B, _ = ops.permute(U[: n * k], (n, k), (1, 0))
             A = ops.empty(m * k, dt)
@@ -513,7 +581,7 @@ def replacebond(psi, b, phi, pdims, ortho, maxdim, mindim, cutoff):
         s = ops.to_host(S)[:k]
         Sd = ops.to_device(np.ravel(np.diag(s), order="F"), dt)
         if ortho == "left":
-            A = U[: m * k]
+            A = U[: m * k].clone()
             B = ops.empty(k * n, dt)
             ops.gemm("N", "T", k, n, k, 1.0, Sd, k, V, n, 0.0, B, k)
             ops.scal(B, 1.0 / ops.nrm2(B))
@@ -548,6 +616,19 @@ class Sweeps:
         self.cutoff = fill(cutoff)
         self.noise = fill(noise)
 
+    def as_dict(self):
+        return {"nsweep": self.nsweep, "maxdim": self.maxdim, "mindim": self.mindim,
+                "cutoff": self.cutoff, "noise": self.noise}
+
+    @staticmethod
+    def from_dict(d):
+        return Sweeps(d["nsweep"], d["maxdim"], d["mindim"], d["cutoff"], d["noise"])
+
+    def tail(self, k):
+        """The schedule from sweep k (0-based) on."""
+        return Sweeps(self.nsweep - k, self.maxdim[k:], self.mindim[k:], self.cutoff[k:],
+                      self.noise[k:])
+
 
 def sweepnext(N):
     for b in range(N - 1):
@@ -557,15 +638,21 @@ def sweepnext(N):
 
 
 def dmrg(H, psi, sweeps, mode="native", krylovdim=3, precond=None, errgoal=1e-14,
-         record=None, sweep_callback=None, env_mode=None, ph=None, timer=None, native=True):
+         record=None, sweep_callback=None, env_mode=None, ph=None, timer=None, native=True,
+         checkpoint=None, first_sweep=0, history=None):
     """Two-site DMRG; psi (right-orthogonal, centre at site 0) is updated in
     place. Returns the final energy. `precond`: None, "diag" or a callable
     factory (PH, b, xdims) -> precond(q, lam). Passing the ProjMPO `ph` of a
     previous call continues from its environments (a completed sweep leaves
     psi right-orthogonal and the right environments current). With
     `native` (default) the per-bond Davidson runs in libitg (itg_davidson2)
-    unless a learned preconditioner is given."""
+    unless a learned preconditioner is given. With `checkpoint` (a path) the
+    MPS and the sweep state (sweeps done, energy, per-sweep records, schedule)
+    are written after every sweep; dmrg_resume continues from such a file.
+    `first_sweep` numbers the sweeps of a resumed schedule and `history` holds
+    the records of the sweeps before it."""
     N = len(psi)
+    history = list(history or [])
     # environments accumulate along the chain (L_b is b successive updates),
     # so their errors add up: they stay FP64 unless asked otherwise
     env_mode = "native" if env_mode is None else env_mode
@@ -580,10 +667,7 @@ def dmrg(H, psi, sweeps, mode="native", krylovdim=3, precond=None, errgoal=1e-14
     for sw in range(sweeps.nsweep):
         t0 = time.perf_counter()
         maxerr, stats = 0.0, {}
-        if sweeps.noise[sw]:
-            # the density-matrix noise term of ITensors is not built; run
-            # without it (noise 1e-10..1e-11 in the reference tests)
-            stats["noise_ignored"] = sweeps.noise[sw]
+        noise = float(sweeps.noise[sw] or 0.0)
         for b, ha in sweepnext(N):
             A, Ad = psi.data[b], psi.dims[b]
             B, Bd = psi.data[b + 1], psi.dims[b + 1]
@@ -596,8 +680,8 @@ def dmrg(H, psi, sweeps, mode="native", krylovdim=3, precond=None, errgoal=1e-14
                 energy, phi = PH.eigsolve(b, phi, pdims, maxiter=krylovdim - 1,
                                           errgoal=errgoal, diag=dg, stats=stats)
             else:
-                work = ops.heff2_work(pdims[0], H.dims[b][0], H.dims[b][1], H.dims[b + 1][1],
-                                      pdims[1], pdims[3], str(phi.dtype).split(".")[-1])
+                work = PH.heff_work((pdims[0], H.dims[b][0], H.dims[b][1], H.dims[b + 1][1],
+                                     pdims[1], pdims[3]))
 
                 def mv(x, b=b, pdims=pdims, work=work):
                     return PH.product(b, x, pdims, work=work)
@@ -612,8 +696,16 @@ def dmrg(H, psi, sweeps, mode="native", krylovdim=3, precond=None, errgoal=1e-14
                                        fallback=getattr(pre, "fallback", None))
             ortho = "left" if ha == 1 else "right"
             ev = timer.start() if timer is not None else None
+            kt = timer is not None and b == timer.factorize_bond and ha == 1 and \
+                timer.factorize_kinds
+            if kt:
+                _lib.ctx().timing_set(tuple(timer.base_kinds) + timer.factorize_kinds)
+            drho = noiseterm(PH, b, phi, pdims, ortho, mode) if noise > 0.0 else None
             terr, k = replacebond(psi, b, phi, pdims, ortho, sweeps.maxdim[sw],
-                                  sweeps.mindim[sw], sweeps.cutoff[sw])
+                                  sweeps.mindim[sw], sweeps.cutoff[sw], drho, noise)
+            if kt:
+                _lib.ctx().timing_set(timer.base_kinds)
+                timer.stop("factorize_timed_bond", timer.start_at(ev))
             if ev is not None:
                 timer.stop("factorize", ev)
             maxerr = max(maxerr, terr)
@@ -641,14 +733,45 @@ def dmrg(H, psi, sweeps, mode="native", krylovdim=3, precond=None, errgoal=1e-14
             PH.mode = saved
             energy = ops.dot(phi, y) / ops.dot(phi, phi)
         _lib.ctx().sync()
-        rec = {"sweep": sw + 1, "energy": energy, "energy_davidson": lam, "maxerr": maxerr,
-               "maxlinkdim": psi.maxlinkdim(), "seconds": time.perf_counter() - t0,
+        rec = {"sweep": first_sweep + sw + 1, "energy": energy, "energy_davidson": lam,
+               "maxerr": maxerr, "maxlinkdim": psi.maxlinkdim(),
+               "seconds": time.perf_counter() - t0,
                "matvecs": stats.get("matvecs", 0), "reverted": stats.get("reverted", 0)}
         if record is not None:
             record.append(rec)
+        history.append(rec)
+        if checkpoint is not None:
+            from .checkpoint import save_mps
+            full = Sweeps(first_sweep + sweeps.nsweep, [None] * first_sweep + sweeps.maxdim,
+                          [None] * first_sweep + sweeps.mindim,
+                          [None] * first_sweep + sweeps.cutoff,
+                          [None] * first_sweep + sweeps.noise).as_dict()
+            save_mps(checkpoint, psi, {"sweeps_done": first_sweep + sw + 1, "energy": energy,
+                                       "mode": mode, "krylovdim": krylovdim,
+                                       "schedule": full, "record": history})
         if sweep_callback is not None:
             sweep_callback(rec)
     return energy
+
+
+def dmrg_resume(H, path, sweeps, **kwargs):
+    """Continue a dmrg run from the checkpoint at `path` (written by
+    dmrg(..., checkpoint=path) after a completed sweep): the MPS is read
+    back, its environments rebuilt, and the sweeps of `sweeps` after those
+    already done are run, checkpointing to the same path. A completed sweep
+    leaves the MPS right-orthogonal with its centre at site 0, the state
+    dmrg starts from, so the continued run repeats the uninterrupted one.
+    Returns (energy, psi)."""
+    from .checkpoint import load_mps
+    psi, meta = load_mps(path)
+    done = int(meta["sweeps_done"])
+    if done >= sweeps.nsweep:
+        return float(meta["energy"]), psi
+    kwargs.setdefault("krylovdim", meta.get("krylovdim", 3))
+    kwargs.setdefault("mode", meta.get("mode", "native"))
+    E = dmrg(H, psi, sweeps.tail(done), checkpoint=path, first_sweep=done,
+             history=meta.get("record"), **kwargs)
+    return E, psi
 
 
 def expect_energy(H, psi):

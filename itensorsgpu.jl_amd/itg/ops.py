@@ -237,8 +237,17 @@ def qr(A, m, n):
 
 
 # ------------------------------------------------------------------- Heff
-def heff2_work(chiL, wl, wm, wr, d, chiR, dtype="float64"):
-    return empty(lib().itg_heff2_work_elems(chiL, wl, wm, wr, d, chiR), dtype)
+def heff2_work_bytes(chiL, wl, wm, wr, d, chiR):
+    """Bytes of the Heff work buffer (itg_heff2_work_elems counts 4-byte
+    units, enough for FP64 and FP32 applies alike)."""
+    return 4 * int(lib().itg_heff2_work_elems(chiL, wl, wm, wr, d, chiR))
+
+
+def heff2_work(chiL, wl, wm, wr, d, chiR, dtype=None):
+    """Work buffer of one Heff apply, allocated in the 4-byte units the
+    library counts it in (`dtype` is accepted for the FP64/FP32 callers and
+    does not change the size)."""
+    return empty(lib().itg_heff2_work_elems(chiL, wl, wm, wr, d, chiR), "float32")
 
 
 def heff2_apply(L, W1, W2, R, psi, chiL, wl, wm, wr, d, chiR, mode="native", out=None,

@@ -11,6 +11,7 @@
 module ITensorsROC
 
 using ITensors, ITensors.NDTensors, LinearAlgebra
+import ITensors: randn!
 import ITensors.NDTensors: contract!!, permutedims!!, similartype, truncate!
 import Base: +, -, *, /, permute!
 import LinearAlgebra: norm, svd, eigen, qr
@@ -55,8 +56,31 @@ mutable struct RocVector{T} <: AbstractVector{T}
     end
 end
 Base.size(v::RocVector) = (v.len,)
-Base.getindex(v::RocVector, i::Int) = error("scalar indexing of device storage; use cpu()")
 Base.pointer(v::RocVector) = v.ptr
+
+# scalar read: one element copied to the host (synchronising), as the
+# reference's `collect(data(D))[]` (src/tensor/cudense.jl:25-26)
+function Base.getindex(v::RocVector{T}, i::Int) where {T}
+    checkbounds(v, i)
+    a = Ref{T}()
+    check(ccall((:itg_memcpy_d2h, libitg), Cint, (Ptr{Cvoid}, Ptr{T}, Ptr{T}, Csize_t),
+                ctx(), a, v.ptr + (i - 1) * sizeof(T), sizeof(T)))
+    return a[]
+end
+
+function RocVector{T}(v::RocVector{T}, n::Integer) where {T}   # first n elements, own storage
+    w = RocVector{T}(undef, n)
+    check(ccall((:itg_memcpy_d2d, libitg), Cint, (Ptr{Cvoid}, Ptr{T}, Ptr{T}, Csize_t),
+                ctx(), w.ptr, v.ptr, n * sizeof(T)))
+    return w
+end
+
+function zeros_roc(::Type{T}, n::Integer) where {T}
+    v = RocVector{T}(undef, n)
+    check(ccall((:itg_fill, libitg), Cint, (Ptr{Cvoid}, Cint, Int64, Cdouble, Ptr{T}),
+                ctx(), dtype(T), n, 0.0, v.ptr))
+    return v
+end
 
 function RocVector(a::Vector{T}) where {T}
     v = RocVector{T}(undef, length(a))
@@ -74,6 +98,14 @@ end
 const RocDense{ElT} = Dense{ElT,RocVector{ElT}}
 const RocDenseTensor{ElT,N} = Tensor{ElT,N,<:RocDense{ElT}}
 
+Base.getindex(D::RocDense{<:Number}) = (length(data(D)) == 1 || throw(DimensionMismatch(
+    "scalar read of a storage of length $(length(data(D)))")); data(D)[1])
+Base.getindex(T::RocDenseTensor{<:Number,0}) = store(T)[]
+
+# the generics of src/ITensorsGPU.jl:36-43 / cudense.jl:15-16 for the storage vector
+similartype(::Type{<:RocVector}, eltype::Type) = RocVector{eltype}
+NDTensors.similar(::Type{<:RocVector{T}}, dims) where {T} = RocVector{T}(undef, prod(dims))
+
 # ---------------------------------------------------------------- conversion
 roc(A::ITensor) = ITensor(Dense(RocVector(Vector(data(store(A))))), inds(A))
 cpu(A::ITensor) = storage(A) isa RocDense ? ITensor(Dense(collect(data(storage(A)))), inds(A)) : A
@@ -83,6 +115,66 @@ cpu(ψ::Union{MPS,MPO}) = (ϕ = copy(ψ); for j in eachindex(ϕ) ϕ[j] = cpu(ψ[
 
 function Base.similar(::Type{<:RocDenseTensor{ElT}}, inds) where {ElT}
     return tensor(Dense(RocVector{ElT}(undef, dim(inds))), inds)
+end
+
+# ------------------------------------------------------------------ random
+# Philox Gaussian stream (itg_randn) in place of CURAND randn!
+# (src/cuitensor.jl:35-49, src/mps/cumps.jl:44-53); successive calls draw
+# successive parts of the stream of the current seed.
+const SEED = Ref{UInt64}(0)
+const OFFSET = Ref{UInt64}(0)
+seed!(s::Integer) = (SEED[] = UInt64(s); OFFSET[] = 0; nothing)
+
+function randn!(T::RocDenseTensor{ElT}) where {ElT<:Union{Float32,Float64}}
+    n = dim(T)
+    check(ccall((:itg_randn, libitg), Cint, (Ptr{Cvoid}, Cint, Int64, UInt64, UInt64, Ptr{ElT}),
+                ctx(), dtype(ElT), n, SEED[], OFFSET[], pointer(data(store(T)))))
+    OFFSET[] += n
+    return T
+end
+randn!(T::ITensor) = (randn!(tensor(T)); T)
+
+rocITensor(::Type{T}, inds::Indices) where {T<:Number} =
+    ITensor(Dense(zeros_roc(T, dim(inds))), inds)
+rocITensor(::Type{T}, inds::Index...) where {T<:Number} = rocITensor(T, IndexSet(inds...))
+randomRocITensor(::Type{S}, inds::Indices) where {S<:Real} = randn!(rocITensor(S, inds))
+randomRocITensor(::Type{S}, inds::Index...) where {S<:Real} = randomRocITensor(S, IndexSet(inds...))
+randomRocITensor(inds::Indices) = randomRocITensor(Float64, inds)
+randomRocITensor(inds::Index...) = randomRocITensor(Float64, IndexSet(inds...))
+
+function rocnormalize!(T::ITensor)
+    st = tensor(T)
+    nrm = norm(st)
+    check(ccall((:itg_scal, libitg), Cint, (Ptr{Cvoid}, Cint, Int64, Cdouble, Ptr{eltype(st)}),
+                ctx(), dtype(eltype(st)), dim(st), 1 / nrm, pointer(data(store(st)))))
+    return T
+end
+
+# cuMPS(T, sites; linkdims) builds dimension-1 links (src/mps/cumps.jl:12-38,
+# quirk A10); linkdims is honoured here
+function rocMPS(::Type{T}, sites::Vector{<:Index}; linkdims::Integer=1) where {T<:Number}
+    N = length(sites)
+    N == 1 && return MPS([rocITensor(T, sites[1])])
+    l = [Index(linkdims, "Link,l=$ii") for ii in 1:N-1]
+    v = Vector{ITensor}(undef, N)
+    for ii in 1:N
+        v[ii] = ii == 1 ? rocITensor(T, l[1], sites[1]) :
+                ii == N ? rocITensor(T, l[N-1], sites[N]) :
+                rocITensor(T, l[ii-1], sites[ii], l[ii])
+    end
+    return MPS(v, 0, N + 1)
+end
+rocMPS(sites::Vector{<:Index}; kwargs...) = rocMPS(Float64, sites; kwargs...)
+
+function randomRocMPS(sites::Vector{<:Index}; linkdims::Integer=1)
+    M = rocMPS(sites; linkdims=linkdims)
+    for i in eachindex(sites)
+        randn!(M[i])
+        rocnormalize!(M[i])
+    end
+    M.llim = 1
+    M.rlim = length(M)
+    return M
 end
 
 # ----------------------------------------------------------------- contract
@@ -173,8 +265,9 @@ function svd(T::RocDenseTensor{ElT,2}; maxdim=minimum(dims(T)), mindim=1, cutoff
                 absoluteCutoff, doRelCutoff, k, te, dc))
     s = collect(S)[1:k[]]
     u = Index(k[], "Link,u"); v = Index(k[], "Link,v")
-    Ut = tensor(Dense(U), (ind(T, 1), u))   # leading k columns are used
-    Vt = tensor(Dense(V), (ind(T, 2), v))
+    # the leading k columns, in storage of exactly their length
+    Ut = tensor(Dense(RocVector{ElT}(U, m * k[])), (ind(T, 1), u))
+    Vt = tensor(Dense(RocVector{ElT}(V, n * k[])), (ind(T, 2), v))
     St = tensor(Dense(RocVector(vec(Matrix(Diagonal(s))))), (u, v))
     return Ut, St, Vt, Spectrum(s .^ 2, te[])
 end
@@ -194,7 +287,7 @@ function eigen(T::Hermitian{ElT,<:RocDenseTensor{ElT,2}}; maxdim=minimum(dims(pa
                 doRelCutoff, k, te, dc))
     d = collect(w)[1:k[]]
     l = Index(k[], "Link,eigen"); r = Index(k[], "Link,eigen")
-    Ut = tensor(Dense(U), (dag(ind(P, 2)), dag(r)))
+    Ut = tensor(Dense(RocVector{ElT}(U, n * k[])), (dag(ind(P, 2)), dag(r)))
     Dt = tensor(Diag(d), (l, dag(r)))
     return Dt, Ut, Spectrum(d, te[])
 end
@@ -209,6 +302,114 @@ function qr(T::RocDenseTensor{ElT,2}; kwargs...) where {ElT}
     return tensor(Dense(Q), (ind(T, 1), q)), tensor(Dense(R), (q, ind(T, 2)))
 end
 
-export roc, cpu, RocVector, RocDense
+# -------------------------------------------------- two-site ProjMPO product
+# The library's index orders: L (a', wl, a), W1 (wl, wm, s1', s1),
+# W2 (wm, wr, s2', s2), R (b', wr, b), psi (a, s1, s2, b). An absent
+# environment (chain end) or MPO link counts as dimension 1, which leaves the
+# memory layout of the others unchanged.
+_dim(i) = isnothing(i) ? 1 : dim(i)
+_ord(T, is...) = permute(T, filter(!isnothing, collect(is))...)
+_ptr(T::ITensor) = pointer(data(store(tensor(T))))
+
+function _heff2_operands(L, W1, W2, R, psi)
+    s1 = commonind(W1, psi); s2 = commonind(W2, psi)
+    a = isnothing(L) ? nothing : commonind(L, psi)
+    b = isnothing(R) ? nothing : commonind(R, psi)
+    wl = isnothing(L) ? nothing : commonind(L, W1)
+    wm = commonind(W1, W2)
+    wr = isnothing(R) ? nothing : commonind(W2, R)
+    dims = Int64[_dim(a), _dim(wl), _dim(wm), _dim(wr), dim(s1), _dim(b)]
+    ops = (isnothing(L) ? nothing : _ord(L, prime(a), wl, a), _ord(W1, wl, wm, prime(s1), s1),
+           _ord(W2, wm, wr, prime(s2), s2), isnothing(R) ? nothing : _ord(R, prime(b), wr, b),
+           _ord(psi, a, s1, s2, b))
+    return dims, ops, (a, s1, s2, b)
+end
+
+function _one_env(::Type{ElT}) where {ElT}
+    v = RocVector{ElT}(undef, 1)
+    check(ccall((:itg_fill, libitg), Cint, (Ptr{Cvoid}, Cint, Int64, Cdouble, Ptr{ElT}),
+                ctx(), dtype(ElT), 1, 1.0, v.ptr))
+    return v
+end
+
+"""
+    heff2_product(L, W1, W2, R, psi; mode=0) -> ITensor
+
+`noprime(((psi*L)*W1)*W2)*R`, the ProjMPO product dmrg runs in its
+eigensolver, as one itg_heff2_apply (L or R `nothing` at a chain end).
+mode: 0 FP64, 1 FP32, 2 split-BF16 contractions.
+"""
+function heff2_product(L, W1::ITensor, W2::ITensor, R, psi::ITensor; mode::Integer=0)
+    ElT = eltype(tensor(psi))
+    dims, (Lp, W1p, W2p, Rp, pp), (a, s1, s2, b) = _heff2_operands(L, W1, W2, R, psi)
+    out = rocITensor(ElT, filter(!isnothing, [a, s1, s2, b])...)
+    work = RocVector{Float32}(undef, ccall((:itg_heff2_work_elems, libitg), Int64,
+                                           (Int64, Int64, Int64, Int64, Int64, Int64), dims...))
+    one = _one_env(ElT)
+    Lptr = isnothing(Lp) ? Ptr{ElT}(one.ptr) : _ptr(Lp)
+    Rptr = isnothing(Rp) ? Ptr{ElT}(one.ptr) : _ptr(Rp)
+    GC.@preserve Lp W1p W2p Rp pp out work one begin
+        check(ccall((:itg_heff2_apply, libitg), Cint,
+                    (Ptr{Cvoid}, Cint, Cint, Int64, Int64, Int64, Int64, Int64, Int64, Ptr{ElT},
+                     Ptr{ElT}, Ptr{ElT}, Ptr{ElT}, Ptr{ElT}, Ptr{ElT}, Ptr{Float32}),
+                    ctx(), dtype(ElT), mode, dims..., Lptr, _ptr(W1p), _ptr(W2p), Rptr,
+                    _ptr(pp), _ptr(out), work.ptr))
+    end
+    return out
+end
+
+"""
+    roc_product(P::ProjMPO, v::ITensor; mode=0)
+
+The two-site ProjMPO product through heff2_product, reading the
+environments and MPO tensors the ProjMPO holds at its position.
+"""
+function roc_product(P::ProjMPO, v::ITensor; mode::Integer=0)
+    nsite(P) == 2 || throw(ArgumentError("roc_product: two-site ProjMPO only"))
+    L = ITensors.lproj(P); R = ITensors.rproj(P)
+    return heff2_product(L, P.H[P.lpos + 1], P.H[P.lpos + 2], R, v; mode=mode)
+end
+
+"""
+    roc_davidson(L, W1, W2, R, psi; maxiter=2, errgoal=1e-14, mode=0) -> (λ, ϕ)
+
+ITensors' davidson for the two-site operator on one bond, run in the
+library (itg_heff2_prepare + itg_davidson2): FP64 Krylov basis on the
+device, split-BF16 applies replayed from HIP graphs when mode = 2.
+"""
+function roc_davidson(L, W1::ITensor, W2::ITensor, R, psi::ITensor; maxiter::Integer=2,
+                      errgoal::Real=1e-14, mode::Integer=0)
+    ElT = eltype(tensor(psi))
+    ElT === Float64 || throw(ArgumentError("roc_davidson: Float64 vectors only"))
+    dims, (Lp, W1p, W2p, Rp, pp), (a, s1, s2, b) = _heff2_operands(L, W1, W2, R, psi)
+    phi = ITensor(Dense(RocVector{ElT}(data(store(tensor(pp))), dim(tensor(pp)))), inds(pp))
+    one = _one_env(ElT)
+    Lptr = isnothing(Lp) ? Ptr{ElT}(one.ptr) : _ptr(Lp)
+    Rptr = isnothing(Rp) ? Ptr{ElT}(one.ptr) : _ptr(Rp)
+    prep = RocVector{Float32}(undef, ccall((:itg_heff2_prep_elems, libitg), Int64,
+                                           (Int64, Int64, Int64, Int64, Int64, Int64), dims...))
+    work = RocVector{Float32}(undef, ccall((:itg_heff2_work_elems, libitg), Int64,
+                                           (Int64, Int64, Int64, Int64, Int64, Int64), dims...))
+    kwork = RocVector{Float64}(undef, ccall((:itg_davidson2_work_elems, libitg), Int64,
+                                            (Int64, Int64, Int64, Cint), dims[1], dims[5],
+                                            dims[6], maxiter))
+    λ = Ref{Cdouble}(0); nmv = Ref{Cint}(0)
+    GC.@preserve Lp W1p W2p Rp phi prep work kwork one begin
+        check(ccall((:itg_heff2_prepare, libitg), Cint,
+                    (Ptr{Cvoid}, Cint, Cint, Int64, Int64, Int64, Int64, Int64, Int64, Ptr{ElT},
+                     Ptr{ElT}, Ptr{ElT}, Ptr{ElT}, Ptr{Float32}),
+                    ctx(), dtype(ElT), mode, dims..., Lptr, _ptr(W1p), _ptr(W2p), Rptr, prep.ptr))
+        check(ccall((:itg_davidson2, libitg), Cint,
+                    (Ptr{Cvoid}, Cint, Cint, Int64, Int64, Int64, Int64, Int64, Int64, Ptr{ElT},
+                     Ptr{ElT}, Ptr{Float32}, Ptr{ElT}, Cint, Cdouble, Ptr{ElT}, Ptr{Float32},
+                     Ptr{Float64}, Ref{Cdouble}, Ref{Cint}),
+                    ctx(), dtype(ElT), mode, dims..., Lptr, Rptr, prep.ptr, _ptr(phi), maxiter,
+                    errgoal, C_NULL, work.ptr, kwork.ptr, λ, nmv))
+    end
+    return λ[], phi
+end
+
+export roc, cpu, RocVector, RocDense, seed!, randomRocITensor, rocITensor, rocMPS,
+       randomRocMPS, heff2_product, roc_product, roc_davidson
 
 end # module

@@ -87,6 +87,38 @@ def heff_apply(L, W1, W2, R, psi):
     return np.einsum("ptzvb,qvb->ptzq", T, R)
 
 
+def heff_apply_blas(L, W1, W2, R, psi):
+    """heff_apply in the same contraction order with every step a BLAS
+    matrix product on reshaped column-major views (the CPU baseline's
+    form; equal to heff_apply to rounding)."""
+    chiL, w, _ = L.shape
+    _, d1, d2, chiR = psi.shape
+    wr = W2.shape[1]
+    T1 = np.reshape(L, (chiL * w, chiL), order="F") @ np.reshape(psi, (chiL, -1), order="F")
+    # X[(wl s1 s2), (s1' s2' wr)] = sum_wm W1[wl, wm, s1', s1] W2[wm, wr, s2', s2]
+    X = np.einsum("wmts,mvzu->wsutzv", W1, W2).reshape(w * d1 * d2, d1 * d2 * wr, order="F")
+    T1 = np.reshape(T1, (chiL, w * d1 * d2, chiR), order="F")
+    T3 = np.matmul(X.T[None, :, :], np.moveaxis(T1, 2, 0).transpose(0, 2, 1))  # (b, o, a')
+    T3 = np.transpose(T3, (2, 1, 0))                                            # (a', o, b)
+    T3 = np.reshape(np.reshape(T3, (chiL, d1 * d2, wr, chiR), order="F"),
+                    (chiL * d1 * d2, wr * chiR), order="F")
+    out = T3 @ np.reshape(R, (chiR, wr * chiR), order="F").T
+    return np.reshape(out, (chiL, d1, d2, chiR), order="F")
+
+
+def update_left_blas(L, A, W):
+    """update_left with BLAS matrix products (same order)."""
+    chiL, w, _ = L.shape
+    _, d, c = A.shape
+    wr = W.shape[1]
+    T = np.reshape(L, (chiL * w, chiL), order="F") @ np.reshape(A, (chiL, d * c), order="F")
+    T = np.reshape(T, (chiL, w, d, c), order="F")                  # (l', wl, s, r)
+    T = np.einsum("pwsr,wvts->ptrv", T, W, optimize=True)           # (l', s', r, wr)
+    T = np.reshape(T, (chiL * d, c * wr), order="F")
+    out = np.reshape(A, (chiL * d, c), order="F").T @ T            # (r', (r wr))
+    return np.transpose(np.reshape(out, (c, c, wr), order="F"), (0, 2, 1))
+
+
 def heff_diag(L, W1, W2, R):
     Ld = np.einsum("awa->aw", L)
     Rd = np.einsum("bwb->bw", R)
@@ -140,21 +172,42 @@ def davidson(matvec, phi0, maxiter=2, miniter=1, errgoal=1e-14, northo=2,
     return lam, phi / np.linalg.norm(phi), nmv
 
 
+# ---------------------------------------------------------- noise term
+def noiseterm(L, W1, W2, R, phi, ortho):
+    """Density-matrix perturbation of two-site dmrg with noise (ITensors
+    0.2 `noiseterm(PH, phi, ortho)`, scaled by `noise!` of
+    examples/dmrg.jl:24 / test/dmrg.jl:24): the kept side's half of the
+    projected operator applied to phi, MPO link left open, traced over the
+    other side. left: indices (a, s1); right: (s2, b), s2 fastest."""
+    dl, d1, d2, dr = phi.shape
+    if ortho == "left":
+        nt = np.einsum("pwa,wvts,asub->ptvub", L, W1, phi)
+        N = np.reshape(nt, (dl * d1, -1), order="F")
+        return N @ N.T
+    nt = np.einsum("asub,mvzu,qvb->asmzq", phi, W2, R)
+    N = np.reshape(nt, (-1, d2 * dr), order="F")
+    return N.T @ N
+
+
 # ---------------------------------------------------------- factorisation
-def factorize(phi, ortho, maxdim, mindim, cutoff):
-    """replacebond! core: returns (A_b, A_{b+1}, truncerr, k)."""
+def factorize(phi, ortho, maxdim, mindim, cutoff, drho=None):
+    """replacebond! core: returns (A_b, A_{b+1}, truncerr, k). A density-
+    matrix perturbation `drho` (already scaled by the noise) forces the eigen
+    route on rho + drho (factorize's eigen_perturbation)."""
     dl, d1, d2, dr = phi.shape
     P = np.reshape(phi, (dl * d1, d2 * dr), order="F")
-    use_eigen = cutoff is not None and cutoff >= 1e-12
+    use_eigen = (cutoff is not None and cutoff >= 1e-12) or drho is not None
     if use_eigen:
         if ortho == "left":
-            w, U, terr, _ = eigh_trunc(P @ P.T, maxdim=maxdim, cutoff=cutoff)
+            rho = P @ P.T if drho is None else P @ P.T + drho
+            w, U, terr, _ = eigh_trunc(rho, maxdim=maxdim, cutoff=cutoff)
             k = U.shape[1]
             A = np.reshape(U, (dl, d1, k), order="F")
             B = np.reshape(U.T @ P, (k, d2, dr), order="F")
             B = B / np.linalg.norm(B)
         else:
-            w, U, terr, _ = eigh_trunc(P.T @ P, maxdim=maxdim, cutoff=cutoff)
+            rho = P.T @ P if drho is None else P.T @ P + drho
+            w, U, terr, _ = eigh_trunc(rho, maxdim=maxdim, cutoff=cutoff)
             k = U.shape[1]
             B = np.reshape(U.T, (k, d2, dr), order="F")
             A = np.reshape(P @ U, (dl, d1, k), order="F")
@@ -183,12 +236,14 @@ def sweepnext(N):
 
 
 def dmrg(H, psi, maxdims, cutoffs, mindims=None, krylovdim=3, precond=False,
-         errgoal=1e-14, record=None):
+         errgoal=1e-14, record=None, noises=None):
     """Two-site DMRG. H: list of MPO tensors, psi: right-orthogonal MPS with
-    its norm on site 0 (modified in place). Returns the final energy."""
+    its norm on site 0 (modified in place). Returns the final energy.
+    noises: per-sweep noise of the density-matrix perturbation (0 = none)."""
     N = len(psi)
     nsw = len(maxdims)
     mindims = mindims or [1] * nsw
+    noises = noises or [0.0] * nsw
     one = np.ones((1, 1, 1))
     Ls = [one] + [None] * N
     Rs = [None] * N + [one]
@@ -216,7 +271,10 @@ def dmrg(H, psi, maxdims, cutoffs, mindims=None, krylovdim=3, precond=False,
             energy, phi, _ = davidson(mv, phi, maxiter=krylovdim - 1, errgoal=errgoal,
                                       precond=pre)
             ortho = "left" if ha == 1 else "right"
-            A, B, terr, k = factorize(phi, ortho, maxdims[sw], mindims[sw], cutoffs[sw])
+            drho = None
+            if noises[sw] > 0.0:
+                drho = noises[sw] * noiseterm(L, W1, W2, R, phi, ortho)
+            A, B, terr, k = factorize(phi, ortho, maxdims[sw], mindims[sw], cutoffs[sw], drho)
             psi[b], psi[b + 1] = A, B
             maxerr = max(maxerr, terr)
             if ha == 1:

@@ -153,6 +153,59 @@ def test_dmrg_tfim_reference(itg):
     assert E >= Ex - 1e-9
 
 
+@pytest.mark.parametrize("ortho", ["left", "right"])
+@pytest.mark.parametrize("mode", ["native", "bf16x3"])
+def test_noiseterm_matches_oracle(itg, ortho, mode):
+    """The density-matrix noise term on the device against the oracle's
+    (itself pinned by the operator-sandwich form), at a mid-chain bond with
+    chi 24 x 32: FP64 to 1e-12, split-BF16 to 5e-5 (it is scaled by the
+    noise before it enters the density matrix)."""
+    from itg import ops as dev
+    from itg.mps import ProjMPO, noiseterm, random_mps
+    N, b = 12, 5
+    H = heis_mpo(N)
+    psi = random_mps(N, 2, 32, seed=9)
+    PH = ProjMPO(H, mode)
+    PH.build_right(psi, upto=b + 2)
+    from itg.mps import update_left
+    for j in range(b):
+        L, Ld = PH.L[j]
+        PH.L[j + 1] = update_left(L, Ld, psi.data[j], psi.dims[j], H.data[j], H.dims[j])
+    phi, pd = dev.contract(psi.data[b], psi.dims[b], [1, 2, 3], psi.data[b + 1],
+                           psi.dims[b + 1], [3, 4, 5], [1, 2, 4, 5])
+    drho = dev.to_host(noiseterm(PH, b, phi, pd, ortho, mode))
+    Hh = models.heisenberg_mpo(N)
+    Lh = ops.from_flat(dev.to_host(PH.L[b][0]), PH.L[b][1])
+    Rh = ops.from_flat(dev.to_host(PH.R[b + 2][0]), PH.R[b + 2][1])
+    ph = ops.from_flat(dev.to_host(phi), pd)
+    ref = odmrg.noiseterm(Lh, Hh[b], Hh[b + 1], Rh, ph, ortho)
+    got = drho.reshape(ref.shape, order="F")
+    err = np.abs(got - ref).max() / np.abs(ref).max()
+    print(ortho, mode, pd, err)
+    assert err < (1e-12 if mode == "native" else 5e-5)
+    np.testing.assert_array_equal(got, got.T)
+
+
+def test_dmrg_with_noise_matches_oracle(itg):
+    """Heisenberg N=20 with a decaying noise schedule (noise! of
+    test/dmrg.jl:24): the device run follows the oracle's identical run
+    (energies per sweep within 1e-8) and ends at the ED energy."""
+    from itg.mps import Sweeps, dmrg, random_mps
+    N = 20
+    maxd, noises = [10, 20, 100, 100, 200, 200], [1e-5, 1e-7, 1e-9, 0.0, 0.0, 0.0]
+    rec = []
+    E = dmrg(heis_mpo(N), random_mps(N, 2, 10, seed=0),
+             Sweeps(6, maxdim=maxd, cutoff=[1e-10], noise=noises), krylovdim=2, record=rec)
+    reco = []
+    Eo = odmrg.dmrg(models.heisenberg_mpo(N), odmrg.random_mps(N, 2, 10, seed=0),
+                    maxdims=maxd, cutoffs=[1e-10] * 6, krylovdim=2, noises=noises, record=reco)
+    for r, ro in zip(rec, reco):
+        print(r["energy_davidson"], ro["energy"], r["maxlinkdim"], ro["maxlinkdim"])
+        assert abs(r["energy_davidson"] - ro["energy"]) < 1e-8
+    assert abs(E - Eo) < 1e-8
+    assert E0_N20 - 1e-10 <= E < E0_N20 + 1e-7
+
+
 def test_dmrg_spin_one(itg):
     """test/dmrg.jl:5-29: S=1 Heisenberg N=10, maxdim 10,20,40: E < -12."""
     from itg.mps import MPO, Sweeps, dmrg, heisenberg_opsum, random_mps
@@ -245,3 +298,39 @@ def test_davidson2_graph_replays_match_eager(itg):
         ops.scal(scratch, 1.0001)          # eager work between replays
         _lib.ctx().sync()
         torch.cuda.synchronize()
+
+
+def test_truncating_sweeps_memory_bounded(itg):
+    """Several truncating split-BF16 sweeps (N=40, maxdim ramp 64..256, cutoff
+    1e-9, so nearly every bond has its own (chiL, chiR)): the ProjMPO keeps
+    one grow-only set of work buffers, and the peak of device memory stays
+    under a bound computed from the largest shape (environments, MPS,
+    one Heff work + Krylov basis, eigensolver buffers)."""
+    import torch
+    from itg import _lib, ops
+    from itg.mps import Sweeps, dmrg, random_mps
+    N, chi, w, d = 40, 256, 5, 2
+    torch.cuda.synchronize()
+    base = torch.cuda.memory_allocated()
+    torch.cuda.reset_peak_memory_stats()
+    psi = random_mps(N, 2, 10, seed=4)
+    rec = []
+    dmrg(heis_mpo(N), psi, Sweeps(5, maxdim=[64, 128, 256], cutoff=[1e-9]), mode="bf16x3",
+         record=rec)
+    torch.cuda.synchronize()
+    PH = __import__("itg.mps", fromlist=["dmrg"]).dmrg.last_projmpo
+    shapes = {dm for dm in psi.dims}
+    peak = torch.cuda.max_memory_allocated() - base
+    env = 2 * (N + 1) * chi * w * chi * 8
+    mps = 2 * N * chi * d * chi * 8
+    heff = ops.heff2_work_bytes(chi, w, w, w, d, chi) + \
+        4 * _lib.lib().itg_heff2_prep_elems(chi, w, w, w, d, chi)
+    kw = 8 * _lib.lib().itg_davidson2_work_elems(chi, d, chi, 2)
+    eig = 6 * (d * chi) ** 2 * 8
+    bound = 1.5 * (env + mps + heff + kw + eig) + (64 << 20)
+    print("distinct site shapes %d, peak %.1f MB, bound %.1f MB, buffers %s"
+          % (len(shapes), peak / 1e6, bound / 1e6, sorted(PH._bufs)))
+    assert rec[-1]["maxlinkdim"] == chi
+    assert len(shapes) > 20
+    assert len(PH._bufs) <= 3
+    assert peak < bound

@@ -184,6 +184,24 @@ def test_projmpo_apply_equals_dense_operator():
     np.testing.assert_allclose(dg, np.diag(Hm), atol=1e-13)
 
 
+def test_blas_forms_equal_einsum_forms():
+    """The BLAS-product forms used for the CPU baseline equal the einsum
+    forms (themselves pinned by the dense operator above)."""
+    rng = np.random.default_rng(11)
+    chiL, chiR, w, d, c = 5, 7, 5, 2, 6
+    L = rng.standard_normal((chiL, w, chiL))
+    R = rng.standard_normal((chiR, w, chiR))
+    W1 = rng.standard_normal((w, w, d, d))
+    W2 = rng.standard_normal((w, 4, d, d))
+    R4 = rng.standard_normal((chiR, 4, chiR))
+    psi = rng.standard_normal((chiL, d, d, chiR))
+    np.testing.assert_allclose(odmrg.heff_apply_blas(L, W1, W2, R4, psi),
+                               odmrg.heff_apply(L, W1, W2, R4, psi), rtol=1e-12, atol=1e-12)
+    A = rng.standard_normal((chiL, d, c))
+    np.testing.assert_allclose(odmrg.update_left_blas(L, A, W1), odmrg.update_left(L, A, W1),
+                               rtol=1e-12, atol=1e-12)
+
+
 def test_random_mps_is_right_canonical():
     psi = odmrg.random_mps(8, 2, 6, seed=0)
     for A in psi[1:]:
@@ -220,6 +238,60 @@ def test_dmrg_tfim_reference_test():
     Ex = models.tfim_exact_energy(N)
     assert abs((E - Ex) / Ex) < 1e-2
     assert E >= Ex - 1e-10   # variational
+
+
+def _noise_setup(seed=7):
+    rng = np.random.default_rng(seed)
+    chiL, chiR, w, d = 3, 4, 5, 2
+    L = rng.standard_normal((chiL, w, chiL))
+    R = rng.standard_normal((chiR, w, chiR))
+    W1 = rng.standard_normal((w, w, d, d))
+    W2 = rng.standard_normal((w, w, d, d))
+    phi = rng.standard_normal((chiL, d, d, chiR))
+    return L, W1, W2, R, phi
+
+
+def test_noiseterm_equals_operator_sandwich():
+    """The noise term as sum_w A_w rho A_w^T with the dense half-operators
+    A_w = sum_wl L[:, wl, :] (x) W[wl, w] built from Kronecker products
+    (left), and the mirror image on the right: an independent contraction
+    path through explicit matrices."""
+    L, W1, W2, R, phi = _noise_setup()
+    chiL, d, _, chiR = phi.shape
+    w = L.shape[1]
+    P = np.reshape(phi, (chiL * d, d * chiR), order="F")
+    rho_l = P @ P.T
+    ref = np.zeros_like(rho_l)
+    for wo in range(w):
+        # A[(a' s'), (a s)] = sum_wl L[a', wl, a] W1[wl, wo, s', s]; (a s) index a fastest
+        A = sum(np.kron(W1[wl, wo], L[:, wl, :]) for wl in range(w))
+        ref += A @ rho_l @ A.T
+    np.testing.assert_allclose(odmrg.noiseterm(L, W1, W2, R, phi, "left"), ref,
+                               rtol=1e-12, atol=1e-12 * np.abs(ref).max())
+    rho_r = P.T @ P
+    ref = np.zeros_like(rho_r)
+    for wo in range(w):
+        # B[(s' b'), (s b)] = sum_wr W2[wo, wr, s', s] R[b', wr, b]; s fastest
+        B = sum(np.kron(R[:, wr, :], W2[wo, wr]) for wr in range(w))
+        ref += B @ rho_r @ B.T
+    np.testing.assert_allclose(odmrg.noiseterm(L, W1, W2, R, phi, "right"), ref,
+                               rtol=1e-12, atol=1e-12 * np.abs(ref).max())
+
+
+def test_dmrg_with_noise_reaches_ed():
+    """Noise schedule as in test/dmrg.jl:24 (decaying to zero): the run
+    still ends at the N=12 ED energy, and a small noise changes the basis
+    choice (different truncation at maxdim-limited bonds)."""
+    H = models.heisenberg_mpo(12)
+    psi = odmrg.random_mps(12, 2, 10, seed=0)
+    rec = []
+    E = odmrg.dmrg(H, psi, maxdims=[10, 20, 64, 64, 64, 64], cutoffs=[1e-12] * 6,
+                   noises=[1e-4, 1e-6, 1e-8, 0.0, 0.0, 0.0], record=rec)
+    assert abs(E - (-5.14209063284054)) < 1e-10
+    psi2 = odmrg.random_mps(12, 2, 10, seed=0)
+    rec2 = []
+    odmrg.dmrg(H, psi2, maxdims=[10], cutoffs=[1e-12], record=rec2)
+    assert rec[0]["energy"] != rec2[0]["energy"]
 
 
 def test_davidson_eigen_residual():
