@@ -72,36 +72,42 @@ def run_mode(mode, N, chi, steps, warmup, krylov, budget, share, config3=False,
              timed_kinds=(), factorize_kinds=()):
     """`warmup` untimed sweeps, then up to `steps` timed sweeps at maxdim chi,
     stopping early (after at least one) when the next sweep would end past
-    `share` seconds of timed work or past the process budget. Default start:
-    a random right-canonical MPS with link dims min(2^j, 2^(N-j), chi)
-    (seed 1); its flat spectra keep every bulk bond at the full chi x chi
-    shape under cutoff 1e-12. config3: random MPS with link dims 10 and the
-    ramp 64..1024 before the chi sweeps. The timed region is bracketed by
-    device synchronisation; every sweep's wall time is recorded."""
+    `share` seconds of timed work or past the process budget.
+
+    Default workload: every timed sweep starts from a fresh random
+    right-canonical MPS with link dims min(2^j, 2^(N-j), chi) (seeds 1, 2, ...,
+    the same in every form), whose flat spectra keep every bulk bond at the
+    full chi x chi shapes under cutoff 1e-12; the new state and its right
+    environments are made before the sweep's clock starts. (Sweeping one
+    state on converges it: at cutoff 1e-12 the chain's ground state needs
+    chi ~ 200, so later sweeps would time a far smaller workload.)
+    config3: random MPS with link dims 10 and the ramp 64..1024, then the
+    timed sweeps continue that one state at maxdim chi.
+    The timed region of each sweep is bracketed by device synchronisation."""
     from itg import _lib
     from itg.mps import MPO, PhaseTimer, ProjMPO, Sweeps, dmrg, heisenberg_opsum, random_mps
     import torch
     H = MPO.from_opsum(heisenberg_opsum([(j, j + 1) for j in range(N - 1)]), N)
     t0 = time.perf_counter()
     cutoff = 1e-12
+    ph = None
     if config3:
         psi = random_mps(N, 2, 10, seed=1)
         ramp = ramp_for(chi)
         dmrg(H, psi, Sweeps(len(ramp), maxdim=ramp, cutoff=[cutoff]), mode=mode,
              krylovdim=krylov, sweep_callback=lambda r: log("[%s] ramp" % mode, r))
         ph = dmrg.last_projmpo
-    else:
-        psi = random_mps(N, 2, chi, seed=1)
-        log("[%s] random MPS chi=%d ready (%.1f s)" % (mode, chi, time.perf_counter() - t0))
-        ph = None
     wrec = []
-    for _ in range(warmup):
+    for w in range(warmup):
+        if not config3:
+            psi = random_mps(N, 2, chi, seed=1000 + w)
+            if ph is not None:
+                ph.build_right(psi)
         dmrg(H, psi, Sweeps(1, maxdim=[chi], cutoff=[cutoff]), mode=mode, krylovdim=krylov,
              ph=ph, record=wrec, sweep_callback=lambda r: log("[%s] warmup" % mode, r))
         ph = dmrg.last_projmpo
     if ph is None:
         ph = ProjMPO(H, mode)
-        ph.build_right(psi)
     setup_s = time.perf_counter() - t0
     log("[%s] set-up done (%.1f s)" % (mode, setup_s))
     # the middle bond (full chi x chi shape) has its eigensolver kernels
@@ -113,29 +119,40 @@ def run_mode(mode, N, chi, steps, warmup, krylov, budget, share, config3=False,
     torch.cuda.synchronize()
     _lib.ctx().sync()
     _lib.ctx().timing(timed_kinds)
-    t1 = time.perf_counter()
+    timed = 0.0
+    fresh_s = 0.0
     done = 0
     while done < steps:
+        if not config3:
+            tf = time.perf_counter()
+            _lib.ctx().timing_set(())
+            psi = random_mps(N, 2, chi, seed=1 + done)
+            ph.build_right(psi)
+            _lib.ctx().timing_set(timed_kinds)
+            torch.cuda.synchronize()
+            _lib.ctx().sync()
+            fresh_s += time.perf_counter() - tf
+        t1 = time.perf_counter()
         dmrg(H, psi, Sweeps(1, maxdim=[chi], cutoff=[cutoff]), mode=mode, krylovdim=krylov,
              ph=ph, timer=timer, record=rec,
              sweep_callback=lambda r: log("[%s] timed" % mode, r))
+        torch.cuda.synchronize()
+        _lib.ctx().sync()
+        timed += time.perf_counter() - t1
         ph = dmrg.last_projmpo
         timer.factorize_kinds = ()
         done += 1
-        el = time.perf_counter() - t1
-        per = el / done
-        if el + per > share or budget.left() < per + 30.0:
+        per = timed / done
+        if timed + per > share or budget.left() < per + (per + fresh_s / done) + 30.0:
             break
-    torch.cuda.synchronize()
-    _lib.ctx().sync()
-    t2 = time.perf_counter()
     kinds = tuple(timed_kinds) + tuple(factorize_kinds)
     kernels = {_lib.KERNEL_NAMES[k]: _lib.ctx().timing_report(k) for k in kinds}
     _lib.ctx().timing(())
     E = rec[-1]["energy"]
-    return {"s_per_sweep": (t2 - t1) / done, "steps": done, "warmup": warmup,
-            "timed_s": t2 - t1, "energy": E, "maxlinkdim": psi.maxlinkdim(),
-            "setup_s": setup_s, "phases": timer.report(), "sweeps": rec,
+    return {"s_per_sweep": timed / done, "steps": done, "warmup": warmup,
+            "timed_s": timed, "energy": E, "maxlinkdim": psi.maxlinkdim(),
+            "setup_s": setup_s, "fresh_state_s": fresh_s, "phases": timer.report(),
+            "sweeps": rec,
             "warmup_sweeps": [{k: r[k] for k in ("energy", "seconds", "maxerr", "matvecs")}
                               for r in wrec],
             "kernels": {k: v for k, v in kernels.items() if v["launches"]},
@@ -343,12 +360,14 @@ def sweep_cost_units(N, d, chi, w):
     return 2.0 * tot / full
 
 
-def cpu_baseline(chi, w=5, d=2, krylov=3, seed=0):
+def cpu_baseline(chi, w=5, d=2, krylov=3, seed=0, eig_n=1024):
     """The oracle's BLAS form (numpy FP64, OpenBLAS/LAPACK) on one bulk bond
-    update at bond dimension chi, the same shapes as the GPU sweep's bulk
-    bonds: `krylov` Heff applies, the density matrix and its
-    eigendecomposition (LAPACK syevd), the new site tensors and one
-    environment update. Returns seconds."""
+    update at bond dimension chi, at the sweep's own shapes: one Heff apply
+    (timed, counted `krylov` times), the density matrix P P^T, the
+    eigendecomposition of an n = eig_n density matrix (LAPACK syevd; timed at
+    that size and scaled by (2 chi / eig_n)^3, since syevd at n = 2 chi runs
+    for minutes on the host) and one environment update. Returns (seconds of
+    one bond update, description of the parts)."""
     import numpy as np
     from oracle import dmrg as odmrg
     rng = np.random.default_rng(seed)
@@ -356,20 +375,32 @@ def cpu_baseline(chi, w=5, d=2, krylov=3, seed=0):
     R = rng.standard_normal((chi, w, chi))
     W = rng.standard_normal((w, w, d, d))
     psi = rng.standard_normal((chi, d, d, chi))
+    parts = {}
     t0 = time.perf_counter()
-    x = psi
-    for i in range(krylov):
-        x = odmrg.heff_apply_blas(L, W, W, R, x)
-        x = x / np.linalg.norm(x)
-        log("cpu baseline: Heff apply %d done (%.1f s)" % (i + 1, time.perf_counter() - t0))
-    P = np.reshape(x, (chi * d, d * chi), order="F")
-    ev, U = np.linalg.eigh(P @ P.T)
-    U = U[:, ::-1][:, :chi]
-    A = np.reshape(U, (chi, d, chi), order="F")
-    B = U.T @ P
-    log("cpu baseline: eigh n=%d done (%.1f s)" % (chi * d, time.perf_counter() - t0))
+    x = odmrg.heff_apply_blas(L, W, W, R, psi)
+    parts["heff_apply"] = time.perf_counter() - t0
+    log("cpu baseline: Heff apply at chi=%d: %.1f s" % (chi, parts["heff_apply"]))
+    t0 = time.perf_counter()
+    P = np.reshape(x / np.linalg.norm(x), (chi * d, d * chi), order="F")
+    rho = P @ P.T
+    parts["density_matrix"] = time.perf_counter() - t0
+    log("cpu baseline: density matrix n=%d: %.1f s" % (chi * d, parts["density_matrix"]))
+    n = min(eig_n, chi * d)
+    t0 = time.perf_counter()
+    ev, U = np.linalg.eigh(rho[:n, :n])
+    parts["eigh_sample"] = time.perf_counter() - t0
+    parts["eigh_scaled"] = parts["eigh_sample"] * (chi * d / n) ** 3
+    log("cpu baseline: eigh n=%d: %.1f s (scaled to n=%d: %.1f s)"
+        % (n, parts["eigh_sample"], chi * d, parts["eigh_scaled"]))
+    Q, _ = np.linalg.qr(rng.standard_normal((chi * d, chi)))
+    A = np.reshape(Q, (chi, d, chi), order="F")
+    t0 = time.perf_counter()
     odmrg.update_left_blas(L, A, W)
-    return time.perf_counter() - t0
+    parts["environment"] = time.perf_counter() - t0
+    log("cpu baseline: environment update: %.1f s" % parts["environment"])
+    bond = (krylov * parts["heff_apply"] + parts["density_matrix"] + parts["eigh_scaled"] +
+            parts["environment"])
+    return bond, parts
 
 
 def blas_threads():
@@ -502,7 +533,7 @@ def main():
     fk = (L_.K_TRD_SYMV, L_.K_TRD_POST)
     shares = {head: 0.30 * args.budget}
     for m in modes[1:]:
-        shares[m] = 0.12 * args.budget
+        shares[m] = 0.14 * args.budget
     for m in modes:
         res[m] = run_mode(m, args.N, args.chi, args.steps, warm, args.krylov, budget,
                           shares[m], args.config3, timed_kinds=timed if m == head else (),
@@ -525,17 +556,23 @@ def main():
     cpu = None
     if not args.no_cpu_baseline:
         cchi = args.cpu_chi or args.chi
-        if budget.left() > 45:
-            t_bond = cpu_baseline(cchi)
+        if budget.left() > 90:
+            t_bond, parts = cpu_baseline(cchi, krylov=args.krylov)
             units = sweep_cost_units(args.N, 2, args.chi, 5) * (args.chi / cchi) ** 3
             cpu = {"value": t_bond * units, "unit": "s/sweep", "cores": blas_threads(),
                    "kind": "port",
                    "sample": "oracle BLAS form (numpy FP64, OpenBLAS/LAPACK) on one bulk bond "
-                             "update at chi=%d (w=5, d=2): %d Heff applies + density matrix + "
-                             "eigh n=%d + site tensors + one environment update = %.2f s, times "
-                             "%.1f bond equivalents per sweep (the sweep's Heff FLOPs in units "
-                             "of one chi=%d bond)"
-                             % (cchi, args.krylov, 2 * cchi, t_bond, units, args.chi)}
+                             "update at chi=%d (w=5, d=2), timed part by part: Heff apply "
+                             "%.2f s (x%d Krylov vectors), density matrix %.2f s, LAPACK "
+                             "syevd timed at n=%d (%.2f s) and scaled by n^3 to n=%d "
+                             "(%.1f s), environment update %.2f s: %.1f s per bond, times "
+                             "%.1f bond equivalents per sweep (the sweep's Heff FLOPs in "
+                             "units of one full-size bond)"
+                             % (cchi, parts["heff_apply"], args.krylov,
+                                parts["density_matrix"], min(1024, 2 * cchi),
+                                parts["eigh_sample"], 2 * cchi, parts["eigh_scaled"],
+                                parts["environment"], t_bond, units),
+                   "parts_s": parts}
             log("cpu baseline", cpu)
         else:
             notes.append("CPU baseline skipped: %.0f s left" % budget.left())

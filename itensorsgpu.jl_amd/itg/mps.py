@@ -540,7 +540,12 @@ def noiseterm(PH, b, phi, pdims, ortho, mode="native"):
         n, K = d2 * chiR, chiL * d1 * Wd[0]
         drho = ops.empty(n * n, dt)
         ops.gemm("T", "N", n, n, K, 1.0, nt, K, nt, K, 0.0, drho, n, mode=mode)
-    return drho
+        m = n
+    # (X + X^T)/2: the split-BF16 product sums its three partial products
+    # in a different order for (i, j) and (j, i), so only this makes the
+    # term exactly symmetric in every mode
+    sym, _ = ops.permute(drho, (m, m), (1, 0), alpha=0.5)
+    return ops.add_permuted(drho, (m, m), (0, 1), 0.5, sym, 1.0)
 
 
 def replacebond(psi, b, phi, pdims, ortho, maxdim, mindim, cutoff, drho=None, noise=0.0):

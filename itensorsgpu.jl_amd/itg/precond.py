@@ -151,10 +151,18 @@ class MLPPreconditioner:
         sn = ops.nrm2(sv)
         return sv, (np.sqrt(sv.numel()) / sn if sn > 0 else 0.0)
 
-    def rq_step(self, matvec, x, lam, q, diag, xdims, X=None, lr=1e-3):
-        """One Adam step on the Ritz value of span{x, t}, t = correction(q):
-        x a unit vector, lam = x.Hx, q = Hx - lam x. Returns mu - lam (the
-        energy the step gains, <= 0). X: the feature rows of q, if at hand."""
+    def rq_grad(self, matvec, x, lam, q, diag, xdims, X=None, backward=True):
+        """Ritz value mu of span{x, t}, t = correction(q), for a unit vector
+        x with lam = x.Hx and q = Hx - lam x; with `backward`, also the
+        parameter gradient into self.grad. Returns (mu, gscale): self.grad
+        is dmu/dparams / gscale (gscale > 0; 0 when the step is degenerate
+        and self.grad is not written). X: the feature rows of q, if at hand.
+
+        With u = ax x + c t the unit Ritz vector (coefficients stationary),
+        dmu/dt = 2c (Hu - mu u); gathered into Y's layout it is dmu/dY. The
+        backward pass differentiates mean((Y - T)^2), whose gradient in Y is
+        2 (Y - T) / (rows out): with T = Y - G/|G| it is G / (|G| rows out / 2),
+        so self.grad = dmu/dparams / (|G| rows out / 2)."""
         chiL, d, _, chiR = xdims
         rows = chiL * chiR
         if X is None:
@@ -172,7 +180,7 @@ class MLPPreconditioner:
         xt, tt, qt, tHt = ops.dot(x, t), ops.dot(t, t), ops.dot(q, t), ops.dot(t, Ht)
         n2 = tt - xt * xt
         if not n2 > 1e-24 * max(tt, 1e-300):
-            return 0.0
+            return lam, 0.0
         tn = np.sqrt(n2)
         # the 2x2 projected problem on {x, e}, e = (t - (x.t) x)/|t_perp|
         xHt = qt + lam * xt
@@ -180,26 +188,36 @@ class MLPPreconditioner:
         xhe = qt / tn
         ev, U = np.linalg.eigh(np.array([[lam, xhe], [xhe, ehe]]))
         mu, a, b = float(ev[0]), float(U[0, 0]), float(U[1, 0])
+        if not backward:
+            return mu, 0.0
         c = b / tn
         ax = a - c * xt
-        # g = dmu/dt = 2c (Hu - mu u), u = ax x + c t, Hx = q + lam x
+        # g = dmu/dt = 2c (Hu - mu u), Hu = ax (q + lam x) + c Ht: the factor
+        # 2c carries the sign of the eigenvector eigh happened to return
         g = self._buf("g", q.numel(), "float64")
-        ops.axpby(c, Ht, 0.0, g)
-        ops.axpby(ax, q, 1.0, g)
-        ops.axpby(ax * (lam - mu), x, 1.0, g)
-        ops.axpby(-mu * c, t, 1.0, g)
+        f = 2.0 * c
+        ops.axpby(f * c, Ht, 0.0, g)
+        ops.axpby(f * ax, q, 1.0, g)
+        ops.axpby(f * ax * (lam - mu), x, 1.0, g)
+        ops.axpby(-f * mu * c, t, 1.0, g)
         G = ops.mlp_gather(g, 1.0, chiL, d, chiR, 0, 1, rows, T=self._buf("G", rows * self.o))
         gn = ops.nrm2(G)
         if not gn > 0:
-            return mu - lam
-        # mlp_backward differentiates mean((Y - T)^2): with T = Y - G/|G| the
-        # parameter gradient is the chain rule of dmu/dY, normalised per step
+            return mu, 0.0
         Tt = self._buf("Trq", rows * self.o)
         ops.axpby(1.0, Y, 0.0, Tt)
         ops.axpby(-1.0 / gn, G, 1.0, Tt)
         ops.mlp_backward(self.params, X, Tt, acts, Y, rows, self.i, self.h, self.o, G=self.grad)
-        self.step += 1
-        ops.mlp_adam(self.params, self.grad, self.m, self.v, self.step, lr=lr)
+        return mu, gn * rows * self.o / 2.0
+
+    def rq_step(self, matvec, x, lam, q, diag, xdims, X=None, lr=1e-3):
+        """One Adam step down the Ritz value of span{x, t} (rq_grad; the
+        gradient is normalised per step). Returns mu - lam (the energy the
+        step gains, <= 0)."""
+        mu, gscale = self.rq_grad(matvec, x, lam, q, diag, xdims, X=X)
+        if gscale > 0:
+            self.step += 1
+            ops.mlp_adam(self.params, self.grad, self.m, self.v, self.step, lr=lr)
         return mu - lam
 
     def factory(self, guard=True, min_residual=0.0):

@@ -45,6 +45,30 @@ def forward(p, X, i, h, o):
     return Y, (Z1, A1, Z2, A2)
 
 
+def bf16_round(x):
+    """Round float32 values to bfloat16 (round to nearest, ties to even),
+    returned as float32: the conversion the fused inference kernel makes
+    (`(__bf16)float` in mlp.hip's mlp_infer_kernel). Finite inputs only."""
+    u = np.ascontiguousarray(x, dtype=np.float32).view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000
+    return u.astype(np.uint32).view(np.float32)
+
+
+def forward_bf16_hidden(p, X, i, h, o):
+    """The fused inference kernel's arithmetic (itg_mlp_infer at h = 256):
+    layer 1 in FP32, its GELU output rounded to bf16; the hidden product with
+    W2 rounded to bf16 and exact (FP32-accumulated on the device) sums; bias,
+    GELU and layer 3 as in `forward`. Evaluated in FP64 from the rounded
+    operands."""
+    W1, b1, W2, b2, W3, b3 = unpack(p, i, h, o)
+    X = np.asarray(X, dtype=np.float64)
+    Z1 = (X @ W1.T + b1).astype(np.float32)
+    A1 = bf16_round(gelu(Z1.astype(np.float64)).astype(np.float32)).astype(np.float64)
+    W2r = bf16_round(W2.astype(np.float32)).astype(np.float64)
+    Z2 = A1 @ W2r.T + b2
+    return gelu(Z2) @ W3.T + b3
+
+
 def loss_and_grad(p, X, T, i, h, o):
     W1, b1, W2, b2, W3, b3 = unpack(p, i, h, o)
     Y, (Z1, A1, Z2, A2) = forward(p, X, i, h, o)

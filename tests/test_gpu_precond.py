@@ -108,6 +108,73 @@ def test_mlp_infer_matches_forward(itg, rows, h, tol):
     assert np.linalg.norm(a - b) <= tol * np.linalg.norm(b)
 
 
+@pytest.mark.parametrize("rows,i,o", [(1000, 12, 4), (130, 27, 9), (64, 5, 1)])
+def test_mlp_infer_fused_vs_oracle_emulation(itg, rows, i, o):
+    """The fused h = 256 inference kernel against the oracle's restatement of
+    its arithmetic (bf16-rounded layer-1 activations and W2, FP32 elsewhere;
+    oracle.mlp.forward_bf16_hidden), in the Frobenius norm: at least ten times
+    closer to it than to the exact forward. (Elementwise the two can differ
+    by one bf16 step where a layer-1 activation lies within FP32 rounding of
+    a bf16 rounding boundary: the kernel's FP32 GELU and the oracle's FP64
+    one may round it to neighbouring bf16 values.)"""
+    from itg import ops
+    rng = np.random.default_rng(rows + i)
+    h = 256
+    p = (rng.standard_normal(omlp.param_count(i, h, o)) * 0.2).astype(np.float32)
+    X = rng.standard_normal((rows, i)).astype(np.float32)
+    Yi = ops.mlp_infer(ops.to_device(p, "float32"), ops.to_device(X.ravel(), "float32"),
+                       rows, i, h, o)
+    got = ops.to_host(Yi).reshape(rows, o)
+    emu = omlp.forward_bf16_hidden(p, X, i, h, o)
+    exact = omlp.forward(p, X, i, h, o)[0]
+    fro = lambda a, b: float(np.linalg.norm(a - b) / np.linalg.norm(b))
+    e_emu, e_exact = fro(got, emu), fro(got, exact)
+    print("mlp_infer rows=%d in=%d out=%d: vs emulation %.2e, vs exact forward %.2e "
+          "(max-norm %.2e, %.2e)" % (rows, i, o, e_emu, e_exact, _relerr(got, emu),
+                                     _relerr(got, exact)))
+    assert e_emu < 5e-5
+    assert e_exact > 10 * e_emu
+
+
+def test_rq_grad_finite_differences(itg):
+    """The Ritz-value gradient of the learned step (MLPPreconditioner.rq_grad,
+    which train_sweep descends) against central differences of the Ritz
+    value itself along random parameter directions, on a Heisenberg bond's
+    own Heff: the directional derivatives agree in sign and to ~1%."""
+    from itg import ops
+    from itg.precond import MLPPreconditioner
+    (mv, phi, dg, pdims), = _chain_operators(itg, 10, 8, 4, {4})
+    x = ops.copy(phi)
+    ops.scal(x, 1.0 / ops.nrm2(x))
+    Hx = ops.copy(mv(x))
+    lam = ops.dot(x, Hx)
+    q = ops.copy(Hx)
+    ops.axpby(-lam, x, 1.0, q)
+    rng = np.random.default_rng(5)
+    net = MLPPreconditioner(2, hidden=32, seed=6)
+    # a network away from its zero-initialised last layer
+    net.set_params(rng.standard_normal(net.nparam).astype(np.float32) * 0.3)
+    p0 = net.host_params().astype(np.float64)
+    mu0, gscale = net.rq_grad(mv, x, lam, q, dg, pdims)
+    assert gscale > 0 and mu0 < lam
+    grad = ops.to_host(net.grad).astype(np.float64) * gscale
+    for trial in range(4):
+        delta = rng.standard_normal(p0.size)
+        eps = 1e-3 * np.linalg.norm(p0) / np.linalg.norm(delta)
+        pp = (p0 + eps * delta).astype(np.float32)
+        pm = (p0 - eps * delta).astype(np.float32)
+        net.set_params(pp)
+        mup = net.rq_grad(mv, x, lam, q, dg, pdims, backward=False)[0]
+        net.set_params(pm)
+        mum = net.rq_grad(mv, x, lam, q, dg, pdims, backward=False)[0]
+        fd = mup - mum
+        an = float(grad @ (pp.astype(np.float64) - pm.astype(np.float64)))
+        print("rq_grad trial %d: finite difference %.6e, gradient %.6e" % (trial, fd, an))
+        assert fd * an > 0
+        assert abs(fd - an) <= 2e-2 * abs(an)
+    net.set_params(p0.astype(np.float32))
+
+
 def test_mlp_training_lowers_loss(itg):
     from itg import ops
     from itg.precond import MLPPreconditioner

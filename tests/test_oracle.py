@@ -365,3 +365,40 @@ def test_mlp_features_elementwise_and_invariances():
     v = rng.standard_normal(n)
     Y = omlp.gather(v, 1.0, chiL, d, chiR)
     np.testing.assert_array_equal(omlp.scatter(Y, chiL, d, chiR, 0, chiL * chiR, np.zeros(n)), v)
+
+
+def test_bf16_round_known_values():
+    """Round-to-nearest-even to 8 significant bits: exact values stay, the
+    halfway cases go to the even neighbour, anything past halfway goes up."""
+    x = np.array([1.0, 1 + 2 ** -8, 1 + 3 * 2 ** -8, 1 + 2 ** -7, 1 + 2 ** -8 + 2 ** -20, -2.5,
+                  -(1 + 2 ** -8 + 2 ** -20), 0.0], dtype=np.float32)
+    want = [1.0, 1.0, 1 + 2 ** -6, 1 + 2 ** -7, 1 + 2 ** -7, -2.5, -(1 + 2 ** -7), 0.0]
+    assert omlp.bf16_round(x).tolist() == want
+    rng = np.random.default_rng(0)
+    y = rng.standard_normal(10000).astype(np.float32) * 100
+    r = omlp.bf16_round(y)
+    assert np.all(np.abs(r - y) <= 2.0 ** -8 * np.abs(y))   # unit roundoff, 8-bit significand
+    # idempotent, and the low 16 bits are clear
+    np.testing.assert_array_equal(omlp.bf16_round(r), r)
+    assert not np.any(r.view(np.uint32) & 0xFFFF)
+
+
+def test_forward_bf16_hidden_is_forward_on_bf16_exact_operands():
+    """With W2 already bf16-exact and layer 1 giving bf16-exact activations
+    (W1 = 0, b1 chosen so GELU(b1) is exact in bf16: b1 = 0 gives 0), the
+    emulation equals the exact forward; with general operands it differs at
+    the bf16 level (~1e-3), not more."""
+    rng = np.random.default_rng(1)
+    i, h, o, rows = 12, 256, 4, 50
+    p = rng.standard_normal(omlp.param_count(i, h, o)) * 0.3
+    X = rng.standard_normal((rows, i))
+    Y = omlp.forward(p, X, i, h, o)[0]
+    Yb = omlp.forward_bf16_hidden(p, X, i, h, o)
+    d = np.abs(Yb - Y).max() / np.abs(Y).max()
+    assert 1e-5 < d < 2e-2, d
+    W1, b1, W2, b2, W3, b3 = omlp.unpack(p, i, h, o)
+    q = np.concatenate([np.zeros(h * i), np.zeros(h),
+                        omlp.bf16_round(W2.astype(np.float32)).astype(np.float64).ravel(),
+                        b2, W3.ravel(), b3])
+    np.testing.assert_allclose(omlp.forward_bf16_hidden(q, X, i, h, o),
+                               omlp.forward(q, X, i, h, o)[0], rtol=1e-13, atol=1e-13)
