@@ -256,6 +256,23 @@ int itg_davidson2(itg_ctx* ctx, int dtype, int mode, int64_t chiL, int64_t wl,
                   const void* R, const void* prep, void* psi, int maxiter,
                   double errgoal, const void* diag, void* work, void* kwork,
                   double* energy, int* matvecs);
+/* No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc). The same product as itg_heff2_apply, computed by the
+ * same kernels, with no `work` argument: the work buffer and any GEMM
+ * split-K partials live in a fixed library-owned device array of
+ * itg_heff2_noalloc_max_bytes() bytes (64 MiB; one copy per GPU, held until
+ * the process ends), so the call makes no hipMalloc/hipFree. Shapes whose work
+ * buffer exceeds it (about chi > 380 at w = 5, d = 2) fail with
+ * ITG_EUNSUPPORTED. Every call on one GPU shares that array: calls that
+ * overlap on one GPU (two streams, two host threads) race. Per-kernel event
+ * timing is off during the call. */
+int64_t itg_heff2_noalloc_max_bytes(void);
+int itg_heff2_apply_noalloc(itg_ctx* ctx, int dtype, int mode, int64_t chiL,
+                            int64_t wl, int64_t wm, int64_t wr, int64_t d,
+                            int64_t chiR, const void* L, const void* W1,
+                            const void* W2, const void* R, const void* psi,
+                            void* out);
 /* Diagonal of the same operator, shape (chiL,d,d,chiR). */
 int itg_heff2_diag(itg_ctx* ctx, int dtype, int64_t chiL, int64_t wl,
                    int64_t wm, int64_t wr, int64_t d, int64_t chiR,

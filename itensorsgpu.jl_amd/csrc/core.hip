@@ -54,6 +54,13 @@ extern "C" int itg_init(int device, itg_ctx** out) {
     delete ctx;
     return ITG_EOOM;
   }
+  if (itg_noalloc_setup(ctx) != ITG_OK) {
+    hipFree(ctx->red);
+    hipHostFree(ctx->red_host);
+    hipStreamDestroy(ctx->own_stream);
+    delete ctx;
+    return ITG_EHIP;
+  }
   *out = ctx;
   return ITG_OK;
 }

@@ -611,6 +611,11 @@ static int launch_gett_t(itg_ctx* ctx, GettArgs& g, bool ak, bool bk) {
     int64_t maxs = g.K / (TR::BKT * 4);
     nsplit = (int)std::max<int64_t>(1, std::min<int64_t>({want, maxs, 64}));
   }
+  if (nsplit > 1 && ctx->fixed_ws) {
+    // a no-alloc call: no more splits than its fixed scratch holds
+    const size_t per = (size_t)g.M * g.N * sizeof(CT);
+    nsplit = (int)std::max<size_t>(1, std::min<size_t>((size_t)nsplit, ctx->fixed_ws_bytes / per));
+  }
   g.partial = nullptr;
   g.kchunk = g.K;
   if (nsplit > 1) {
@@ -619,8 +624,12 @@ static int launch_gett_t(itg_ctx* ctx, GettArgs& g, bool ak, bool bk) {
     nsplit = (int)((g.K + chunk - 1) / chunk);
     g.kchunk = chunk;
     void* ws;
-    int rc = itg_workspace(ctx, (size_t)nsplit * g.M * g.N * sizeof(CT), &ws);
-    if (rc) return rc;
+    if (ctx->fixed_ws) {
+      ws = ctx->fixed_ws;
+    } else {
+      int rc = itg_workspace(ctx, (size_t)nsplit * g.M * g.N * sizeof(CT), &ws);
+      if (rc) return rc;
+    }
     g.partial = ws;
   }
   dim3 grid((unsigned)tiles, 1, (unsigned)nsplit);

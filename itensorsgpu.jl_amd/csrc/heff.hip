@@ -15,6 +15,8 @@
 // split-operand GEMM (bf16x3.hip).
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+
 #include "bf16x3.h"
 #include "itg_internal.h"
 
@@ -329,4 +331,88 @@ extern "C" int itg_heff2_diag(itg_ctx* ctx, int dtype, int64_t chiL, int64_t wl,
         (const float*)R, (float*)diag);
   ITG_LAUNCH_CHECK(ctx);
   return ITG_OK;
+}
+
+// ------------------------------------------------------ no-alloc variant
+// itg_heff2_apply_noalloc keeps its work buffer (T1, T3, the split planes and
+// the prepared operator) and any GEMM split-K partials in `scratch`, a
+// file-scope device array: the HIP runtime gives every GPU its own copy when
+// the library's code object is loaded there, and it is held until the
+// process ends. In host code `scratch` names a host-side stand-in, not device
+// memory, so host code never passes it to a kernel or a copy:
+// itg_noalloc_setup resolves each device's address once, with that device
+// current, and every later use goes through that address.
+constexpr size_t kNoallocScratchBytes = (size_t)64 << 20;
+static __device__ __align__(256) unsigned char scratch[kNoallocScratchBytes];
+
+namespace {
+constexpr int kNoallocMaxDevices = 64;
+std::mutex g_noalloc_mu;
+void* g_noalloc_ptr[kNoallocMaxDevices] = {};
+
+// fixed split-K scratch and no event timing for the duration of one call
+struct NoallocScope {
+  itg_ctx* ctx;
+  int timing;
+  NoallocScope(itg_ctx* c, void* ws, size_t bytes) : ctx(c), timing(c->timing) {
+    ctx->fixed_ws = ws;
+    ctx->fixed_ws_bytes = bytes;
+    ctx->timing = 0;
+  }
+  ~NoallocScope() {
+    ctx->fixed_ws = nullptr;
+    ctx->fixed_ws_bytes = 0;
+    ctx->timing = timing;
+  }
+};
+}  // namespace
+
+int itg_noalloc_setup(itg_ctx* ctx) {
+  if (ctx->device < 0 || ctx->device >= kNoallocMaxDevices)
+    return itg_fail(ctx, ITG_EUNSUPPORTED, "no-alloc scratch: device %d out of range", ctx->device);
+  std::lock_guard<std::mutex> lock(g_noalloc_mu);
+  void* p = g_noalloc_ptr[ctx->device];
+  if (!p) {
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return itg_fail(ctx, ITG_EHIP, "no-alloc scratch address on device %d: %s", ctx->device,
+                      hipGetErrorString(e));
+    g_noalloc_ptr[ctx->device] = p;
+  }
+  ctx->noalloc_scratch = p;
+  return ITG_OK;
+}
+
+extern "C" int64_t itg_heff2_noalloc_max_bytes(void) { return (int64_t)kNoallocScratchBytes; }
+
+extern "C" int itg_heff2_apply_noalloc(itg_ctx* ctx, int dtype, int mode, int64_t chiL,
+                                       int64_t wl, int64_t wm, int64_t wr, int64_t d,
+                                       int64_t chiR, const void* L, const void* W1,
+                                       const void* W2, const void* R, const void* psi,
+                                       void* out) {
+  ITG_CHECK_CTX(ctx);
+  HeffDims h(chiL, wl, wm, wr, d, chiR);
+  int rc = check_heff(ctx, dtype, mode, h);
+  if (rc) return rc;
+  if (!L || !W1 || !W2 || !R || !psi || !out)
+    return itg_fail(ctx, ITG_EARG, "heff2_apply_noalloc: null pointer");
+  if (!ctx->noalloc_scratch)
+    return itg_fail(ctx, ITG_EHIP, "heff2_apply_noalloc: no scratch on device %d", ctx->device);
+  size_t es = itg_dtype_size(dtype);
+  size_t need = al256(work_bytes(h, es));
+  if (need > kNoallocScratchBytes)
+    return itg_fail(ctx, ITG_EUNSUPPORTED,
+                    "heff2_apply_noalloc: shape (%lld,%lld,%lld,%lld,%lld,%lld) needs %zu bytes "
+                    "of work, more than the %zu-byte no-alloc scratch; use itg_heff2_apply "
+                    "with a work buffer",
+                    (long long)chiL, (long long)wl, (long long)wm, (long long)wr, (long long)d,
+                    (long long)chiR, need, kNoallocScratchBytes);
+  char* base = (char*)ctx->noalloc_scratch;
+  // the rest of the array holds the split-K partials of the two GEMMs
+  NoallocScope scope(ctx, base + need, kNoallocScratchBytes - need);
+  Work w = carve_work(h, es, base);
+  Prep P = carve_prep(h, w.prep);
+  rc = prepare(ctx, dtype, mode, h, L, W1, W2, R, P);
+  if (rc) return rc;
+  return apply(ctx, dtype, mode, h, L, R, P, psi, out, w);
 }

@@ -48,6 +48,13 @@ struct itg_ctx {
   size_t event_next = 0;
   // captured Heff-apply graphs of itg_davidson2 (null: seen once, eager)
   std::unordered_map<std::string, hipGraphExec_t> graphs;
+  // this device's copy of the no-alloc scratch array (heff.hip), resolved
+  // once per device by itg_noalloc_setup
+  void* noalloc_scratch = nullptr;
+  // while set, GEMM split-K partials go here (at most fixed_ws_bytes)
+  // instead of the grow-only workspace: the no-alloc entry points
+  void* fixed_ws = nullptr;
+  size_t fixed_ws_bytes = 0;
 };
 
 constexpr int kRedSlots = 16384;
@@ -114,6 +121,9 @@ int itg_tstart(itg_ctx* ctx, int kind);
 void itg_tstop(itg_ctx* ctx, int t, int kind, double flops, double bytes);
 void itg_timing_free(itg_ctx* ctx);
 void itg_graph_cache_clear(itg_ctx* ctx);
+// Resolves (once per device, with that device current) the device address of
+// the no-alloc scratch array into ctx->noalloc_scratch.
+int itg_noalloc_setup(itg_ctx* ctx);
 // Prepared two-site Heff apply (heff.hip) with dims = {chiL, wl, wm, wr, d,
 // chiR}; validates the shape, launches only.
 int itg_heff2_apply_internal(itg_ctx* ctx, int dtype, int mode, const int64_t* dims,

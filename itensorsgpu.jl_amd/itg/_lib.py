@@ -95,6 +95,9 @@ _SIGS = {
     "itg_qr": (C.c_int, [_ctxp, C.c_int, C.c_int64, C.c_int64, _vp, _vp, _vp]),
     "itg_heff2_work_elems": (C.c_int64, [C.c_int64] * 6),
     "itg_heff2_apply": (C.c_int, [_ctxp, C.c_int, C.c_int] + [C.c_int64] * 6 + [_vp] * 7),
+    "itg_heff2_noalloc_max_bytes": (C.c_int64, []),
+    "itg_heff2_apply_noalloc": (C.c_int, [_ctxp, C.c_int, C.c_int] + [C.c_int64] * 6 +
+                                [_vp] * 6),
     "itg_heff2_diag": (C.c_int, [_ctxp, C.c_int] + [C.c_int64] * 6 + [_vp] * 5),
     "itg_heff2_prep_elems": (C.c_int64, [C.c_int64] * 6),
     "itg_heff2_prepare": (C.c_int, [_ctxp, C.c_int, C.c_int] + [C.c_int64] * 6 + [_vp] * 5),

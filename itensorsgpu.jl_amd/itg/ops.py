@@ -265,6 +265,19 @@ def heff2_apply(L, W1, W2, R, psi, chiL, wl, wm, wr, d, chiR, mode="native", out
     return out
 
 
+def heff2_apply_noalloc(L, W1, W2, R, psi, chiL, wl, wm, wr, d, chiR, mode="native", out=None):
+    """heff2_apply with its work in the library's fixed device scratch (no
+    device allocation by the library; `out` still comes from the caller)."""
+    dt = str(psi.dtype).split(".")[-1]
+    if out is None:
+        out = empty(chiL * d * d * chiR, dt)
+    c = ctx()
+    c.check(lib().itg_heff2_apply_noalloc(c.ptr, dtype_code(psi), _lib.MODES[mode], chiL, wl, wm,
+                                          wr, d, chiR, ptr(L), ptr(W1), ptr(W2), ptr(R), ptr(psi),
+                                          ptr(out)))
+    return out
+
+
 def heff2_prep(chiL, wl, wm, wr, d, chiR):
     """Buffer for itg_heff2_prepare (counted in 4-byte units)."""
     return empty(lib().itg_heff2_prep_elems(chiL, wl, wm, wr, d, chiR), "float32")
