@@ -19,10 +19,12 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdlib>
 #include <cmath>
 #include <vector>
 
 #include "dc_core.h"
+#include "eigh2.h"
 #include "gett.h"
 #include "itg_internal.h"
 #include "truncate_core.h"
@@ -34,6 +36,7 @@ int itg_truncate_device_async(itg_ctx* ctx, const double* P, int64_t n, int64_t 
 namespace {
 
 constexpr int NB = 32;          // tridiagonalisation panel width
+constexpr int TWO_STAGE_MIN = 256;   // n from which the two-stage reduction runs
 constexpr int SMALL_MERGE = 256;
 
 __device__ double block_sum(double v, double* sh) {
@@ -1264,27 +1267,32 @@ static int back_transform(itg_ctx* ctx, int n, int nrefl, int off, int k, const 
   return ITG_OK;
 }
 
-// U (n x k) <- Q U for the tridiagonalisation's Q = H_0 ... H_{n-2}, with
-// the reflectors read in place from A (after bt_clear_upper) and every T
-// formed up front by bt_tall_kernel. Groups of BTG panels go last-to-first:
-// Y = V_G^T U, Gg = V_G^T V_G, the group solve for X2, U -= V_G X2.
+// U (n x k) <- Q U for Q = H_0 ... H_{n-off-1}, reflector j stored in
+// column j of A from row j + off on (the tridiagonalisation: off = 1, its
+// rows 0..j cleared here and every T formed up front by bt_tall_kernel; the
+// two-stage band reduction: off = its bandwidth, A already laid out and
+// Tall given). Groups of BTG panels go last-to-first: Y = V_G^T U,
+// Gg = V_G^T V_G, the group solve for X2, U -= V_G X2.
 static int back_transform_trd(itg_ctx* ctx, int n, int k, double* A, const double* tau,
-                              double* U, double* Tall, double* Y, double* X2, double* Gg) {
-  int nrefl = n - 1;
+                              double* U, double* Tall, double* Y, double* X2, double* Gg,
+                              int off = 1, bool have_t = false) {
+  int nrefl = n - off;
   if (nrefl < 1 || k < 1) return ITG_OK;
-  bt_clear_upper_kernel<<<itg_blocks((int64_t)n * n, 256, 16384), 256, 0, ctx->stream>>>(A, n);
-  int npan = (nrefl + NB - 1) / NB;
-  bt_tall_kernel<<<npan, 256, 0, ctx->stream>>>(A, n, nrefl, tau, Tall);
-  ITG_LAUNCH_CHECK(ctx);
+  if (!have_t) {
+    bt_clear_upper_kernel<<<itg_blocks((int64_t)n * n, 256, 16384), 256, 0, ctx->stream>>>(A, n);
+    int npan = (nrefl + NB - 1) / NB;
+    bt_tall_kernel<<<npan, 256, 0, ctx->stream>>>(A, n, nrefl, tau, Tall);
+    ITG_LAUNCH_CHECK(ctx);
+  }
   const int gsz = BTG * NB;
   int ngrp = (nrefl + gsz - 1) / gsz;
   for (int g = ngrp - 1; g >= 0; --g) {
     int p0 = g * gsz;
     int nbg = std::min(gsz, nrefl - p0);
-    int m2 = n - p0 - 1;
-    const double* VG = A + (size_t)p0 * n + (p0 + 1);   // rows p0+1.., columns p0..p0+nbg
+    int m2 = n - p0 - off;
+    const double* VG = A + (size_t)p0 * n + (p0 + off);   // rows p0+off.., columns p0..p0+nbg
     int rc = itg_gemm_internal(ctx, ITG_F64, ITG_MODE_NATIVE, 'T', 'N', nbg, k, m2, 1.0, VG, n,
-                               U + (p0 + 1), n, 0.0, Y, nbg);
+                               U + (p0 + off), n, 0.0, Y, nbg);
     if (rc) return rc;
     rc = itg_gemm_internal(ctx, ITG_F64, ITG_MODE_NATIVE, 'T', 'N', nbg, nbg, m2, 1.0, VG, n, VG,
                            n, 0.0, Gg, nbg);
@@ -1293,7 +1301,7 @@ static int back_transform_trd(itg_ctx* ctx, int n, int k, double* A, const doubl
         nbg, k, Y, Gg, Tall + (size_t)(p0 / NB) * NB * NB, X2);
     ITG_LAUNCH_CHECK(ctx);
     rc = itg_gemm_internal(ctx, ITG_F64, ITG_MODE_NATIVE, 'N', 'N', m2, k, nbg, -1.0, VG, n, X2,
-                           nbg, 1.0, U + (p0 + 1), n);
+                           nbg, 1.0, U + (p0 + off), n);
     if (rc) return rc;
   }
   return ITG_OK;
@@ -1318,13 +1326,20 @@ extern "C" int itg_eigh_trunc(itg_ctx* ctx, int dtype, int64_t n64, void* Ain, d
   int n = (int)n64;
   size_t nn = (size_t)n * n;
   bool f32 = dtype == ITG_F32;
+  // the two-stage reduction (dense -> band -> tridiagonal) on request
+  // (ITG_EIGH_TWOSTAGE=1, read per call): correct, but at n = 4096 it takes
+  // 113 ms against the one-stage reduction's 77 ms, its bulge chasing and
+  // tall-panel QRs being latency-bound
+  const char* two_env = getenv("ITG_EIGH_TWOSTAGE");
+  const bool two = n >= TWO_STAGE_MIN && two_env && two_env[0] == '1';
   const size_t Tw = (size_t)(n + 63) / 64;   // tile rows of the tiled symv
   size_t bytes = (3 * nn + (f32 ? 2 * nn : 0)) * sizeof(double) + 3 * (size_t)n * NB * 8 +
                  (2 * (size_t)BTG * NB * n + (size_t)BTG * NB * BTG * NB) * 8 +
                  (size_t)((n + NB - 1) / NB) * NB * NB * 8 +
                  (size_t)n * 8 * 26 + 4 * (size_t)n * NB * 8 + 4 * NB * NB * 8 + 64 * 1024 +
                  (Tw * n + Tw * (Tw + 1) / 2 + 64) * 8 + 64 * 256 +
-                 ((size_t)n / POST_ROWS + 1) * 8;
+                 ((size_t)n / POST_ROWS + 1) * 8 +
+                 (two ? eig2::doubles_needed(n) * 8 + 256 : 0);
   void* ws;
   rc = eig_workspace(ctx, bytes, &ws);
   if (rc) return rc;
@@ -1360,13 +1375,16 @@ extern "C" int itg_eigh_trunc(itg_ctx* ctx, int dtype, int64_t n64, void* Ain, d
   double* X2grp = cv.take<double>((size_t)BTG * NB * n);
   double* Ggrp = cv.take<double>((size_t)BTG * NB * BTG * NB);
   double* out3 = cv.take<double>(4);
+  eig2::Bufs eb{};
+  if (two) eig2::carve(n, cv.take<double>(eig2::doubles_needed(n)), &eb);
   if (cv.off > bytes) return itg_fail(ctx, ITG_EOOM, "eigh: workspace layout overflow");
   if (f32) {
     rc = itg_convert(ctx, (int64_t)nn, ITG_F32, Ain, ITG_F64, A);
     if (rc) return rc;
   }
   // 1. tridiagonalise
-  rc = tridiagonalize(ctx, n, A, d, e, tau, VW, Ypart, ab, pnorm, pyv);
+  if (two) rc = eig2::reduce(ctx, n, A, d, e, eb);
+  else rc = tridiagonalize(ctx, n, A, d, e, tau, VW, Ypart, ab, pnorm, pyv);
   if (rc) return rc;
   // 2. divide and conquer
   double *Q, *D;
@@ -1419,7 +1437,14 @@ extern "C" int itg_eigh_trunc(itg_ctx* ctx, int dtype, int64_t n64, void* Ain, d
   // 4. kept eigenvectors: gather, back-transform
   gather_cols_kernel<<<itg_blocks((int64_t)n * k, 256, 16384), 256, 0, ctx->stream>>>(n, (int)k, Q, perm, U);
   ITG_LAUNCH_CHECK(ctx);
-  rc = back_transform_trd(ctx, n, (int)k, A, tau, U, Vbuf, Ygrp, X2grp, Ggrp);
+  if (two) {
+    rc = eig2::apply_q2(ctx, n, (int)k, U, eb);
+    if (rc) return rc;
+    rc = back_transform_trd(ctx, n, (int)k, A, nullptr, U, eb.Tall, Ygrp, X2grp, Ggrp, eig2::SB,
+                            true);
+  } else {
+    rc = back_transform_trd(ctx, n, (int)k, A, tau, U, Vbuf, Ygrp, X2grp, Ggrp);
+  }
   if (rc) return rc;
   if (f32) {
     rc = itg_convert(ctx, (int64_t)n * k, ITG_F64, U, ITG_F32, Uout);

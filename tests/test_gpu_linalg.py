@@ -146,6 +146,31 @@ def test_eigh_truncated_density_matrix(dev, n, maxdim, cutoff):
     assert np.abs(Uh.T @ Uh - np.eye(k)).max() < 1e-12
 
 
+@pytest.mark.parametrize("n,cols", [(257, 257), (600, 300), (1024, 1024), (4096, 2048)])
+def test_eigh_two_stage_reduction(dev, monkeypatch, n, cols):
+    """The two-stage reduction (dense -> band of width 32 -> tridiagonal by
+    bulge chasing, then the two back-transformations) against LAPACK on a
+    density matrix M M^T: full rank, and with a (n - cols)-fold cluster at
+    zero; every eigenvalue to 1e-13 of the largest, the kept vectors'
+    eigen-residual and orthonormality."""
+    monkeypatch.setenv("ITG_EIGH_TWOSTAGE", "1")
+    rng = np.random.default_rng(n + cols)
+    M = rng.standard_normal((n, cols))
+    M /= np.linalg.norm(M)
+    rho = M @ M.T
+    maxdim = n // 2
+    w, U, k, te, dc = dev.eigh_trunc(up(dev, rho), n, maxdim=maxdim, cutoff=1e-12)
+    wr, Ur, ter, dcr = olin.eigh_trunc(rho, maxdim=maxdim, cutoff=1e-12)
+    assert k == Ur.shape[1]
+    scale = wr[0]
+    wh = dev.to_host(w)[:k]
+    assert np.abs(wh - wr).max() <= 1e-13 * scale
+    assert abs(te - ter) <= n * 1e-16 * scale + 1e-12 * ter
+    Uh = ops.from_flat(dev.to_host(U)[: n * k], (n, k))
+    assert np.abs(Uh.T @ Uh - np.eye(k)).max() < 1e-12
+    assert np.abs(rho @ Uh - Uh * wh).max() < 1e-13 * scale
+
+
 def test_eigh_degenerate_multiplets(dev):
     """Exactly degenerate eigenvalues (SU(2) multiplets in Heisenberg density
     matrices): the kept invariant subspace must be exact."""
