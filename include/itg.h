@@ -47,7 +47,17 @@ enum itg_status {
   ITG_ECONV = -7         /* iterative method did not converge                */
 };
 
-enum itg_dtype { ITG_F64 = 0, ITG_F32 = 1 };
+/* ITG_C64: ComplexF64, interleaved (re, im) pairs (Julia's ComplexF64 and
+ * Dense{ComplexF64} layout; the reference parameterises its tensor, contraction,
+ * eigen and davidson tests over it: test/test_cucontract.jl:9,
+ * test/test_cuitensor.jl:10, test/test_cuiterativesolvers.jl:13,
+ * src/tensor/culinearalgebra.jl:84-85). Accepted by itg_fill (imaginary part
+ * 0), itg_convert (to and from ITG_F64: real part / zero imaginary part),
+ * itg_randn, itg_permute, itg_add_permuted, itg_scal, itg_axpby (real
+ * coefficients), itg_dot (conj(x).y into out[0], out[1]), itg_nrm2,
+ * itg_contract (ITG_MODE_NATIVE, ranks up to ITG_MAX_RANK - 2) and
+ * itg_eigh_trunc (Hermitian); counts n are complex elements. */
+enum itg_dtype { ITG_F64 = 0, ITG_F32 = 1, ITG_C64 = 2 };
 enum itg_mode { ITG_MODE_NATIVE = 0, ITG_MODE_F32 = 1, ITG_MODE_BF16X3 = 2 };
 
 /* truncate semantics: CPU NDTensors (default) or the reference GPU kernel
@@ -135,7 +145,16 @@ int itg_axpby(itg_ctx* ctx, int dtype, int64_t n, double alpha, const void* x,
  * counterpart in the reference, whose davidson is unpreconditioned. */
 int itg_precond_diag(itg_ctx* ctx, int dtype, int64_t n, const void* q,
                      const void* diag, double lambda, double floor, void* t);
-/* FP64-accumulated dot and 2-norm returned to the host (cudense.jl:25-27). */
+/* x *= (re + i im), x ComplexF64 (n complex elements). */
+int itg_zscal(itg_ctx* ctx, int64_t n, double re, double im, void* x);
+/* y = (are + i aim) x + (bre + i bim) y, ComplexF64. */
+int itg_zaxpby(itg_ctx* ctx, int64_t n, double are, double aim, const void* x,
+               double bre, double bim, void* y);
+/* x = conj(x) in place (dag of the storage, cudense.jl's conj!); a no-op for
+ * real dtypes. */
+int itg_conj(itg_ctx* ctx, int dtype, int64_t n, void* x);
+/* FP64-accumulated dot and 2-norm returned to the host (cudense.jl:25-27).
+ * For ITG_C64 the dot is sum conj(x) y and `out` takes two doubles. */
 int itg_dot(itg_ctx* ctx, int dtype, int64_t n, const void* x, const void* y,
             double* out);
 int itg_nrm2(itg_ctx* ctx, int dtype, int64_t n, const void* x, double* out);
@@ -183,7 +202,8 @@ int itg_truncate_device(itg_ctx* ctx, const double* P_dev, int64_t n,
 
 /* -------------------------------------------------------- factorisations */
 /* Symmetric eigendecomposition A = U diag(w) U^T of a column-major n x n
- * symmetric matrix (both triangles stored). Eigenvalues are returned in DESCENDING order
+ * symmetric matrix (both triangles stored); for ITG_C64 the Hermitian
+ * A = U diag(w) U^H (heevd!'s branch, culinearalgebra.jl:84-85), U complex. Eigenvalues are returned in DESCENDING order
  * into `w` (device, FP64, length n) and the matching eigenvectors into the
  * columns of U (device, n x n, dtype). Then truncate! decides k
  * (src/tensor/culinearalgebra.jl:74-108: syevd, reverse, truncate!; mindim

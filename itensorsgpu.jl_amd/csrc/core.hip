@@ -75,6 +75,7 @@ extern "C" int itg_destroy(itg_ctx* ctx) {
   if (ctx->ws) hipFree(ctx->ws);
   if (ctx->eig_ws) hipFree(ctx->eig_ws);
   if (ctx->mlp_ws) hipFree(ctx->mlp_ws);
+  if (ctx->cws) hipFree(ctx->cws);
   if (ctx->red) hipFree(ctx->red);
   if (ctx->red_host) hipHostFree(ctx->red_host);
   hipStreamDestroy(ctx->own_stream);
@@ -251,6 +252,7 @@ __global__ void fill_kernel(int64_t n, T v, T* x) {
 
 extern "C" int itg_fill(itg_ctx* ctx, int dtype, int64_t n, double value, void* x) {
   ITG_CHECK_CTX(ctx);
+  if (dtype == ITG_C64) return itg_c64_fill(ctx, n, value, x);
   ITG_CHECK_DTYPE(ctx, dtype);
   if (n <= 0) return ITG_OK;
   if (dtype == ITG_F64)
@@ -270,6 +272,8 @@ __global__ void convert_kernel(int64_t n, const S* s, D* d) {
 
 extern "C" int itg_convert(itg_ctx* ctx, int64_t n, int sdt, const void* src, int ddt, void* dst) {
   ITG_CHECK_CTX(ctx);
+  if (sdt == ITG_C64 && ddt == ITG_C64) return itg_memcpy_d2d(ctx, dst, src, (size_t)n * 16);
+  if (sdt == ITG_C64 || ddt == ITG_C64) return itg_c64_convert(ctx, n, sdt, src, ddt, dst);
   ITG_CHECK_DTYPE(ctx, sdt);
   ITG_CHECK_DTYPE(ctx, ddt);
   if (n <= 0) return ITG_OK;
@@ -320,6 +324,13 @@ __global__ void randn_kernel(int64_t n, uint64_t seed, uint64_t offset, T* x) {
 extern "C" int itg_randn(itg_ctx* ctx, int dtype, int64_t n, uint64_t seed, uint64_t offset,
                          void* x) {
   ITG_CHECK_CTX(ctx);
+  if (dtype == ITG_C64) {
+    // complex element i = (g[2(offset+i)] + i g[2(offset+i)+1]) / sqrt(2), g
+    // the real stream: unit variance, as randn(ComplexF64)
+    int rc = itg_randn(ctx, ITG_F64, 2 * n, seed, 2 * offset, x);
+    if (rc) return rc;
+    return itg_scal(ctx, ITG_F64, 2 * n, 0.70710678118654752440, x);
+  }
   ITG_CHECK_DTYPE(ctx, dtype);
   if (n <= 0) return ITG_OK;
   if (dtype == ITG_F64)
@@ -350,6 +361,7 @@ __global__ void axpby_kernel(int64_t n, double a, const T* x, double b, T* y) {
 
 extern "C" int itg_scal(itg_ctx* ctx, int dtype, int64_t n, double alpha, void* x) {
   ITG_CHECK_CTX(ctx);
+  if (dtype == ITG_C64) return itg_scal(ctx, ITG_F64, 2 * n, alpha, x);
   ITG_CHECK_DTYPE(ctx, dtype);
   if (n <= 0 || alpha == 1.0) return ITG_OK;
   if (dtype == ITG_F64)
@@ -363,6 +375,7 @@ extern "C" int itg_scal(itg_ctx* ctx, int dtype, int64_t n, double alpha, void* 
 extern "C" int itg_axpby(itg_ctx* ctx, int dtype, int64_t n, double alpha, const void* x,
                          double beta, void* y) {
   ITG_CHECK_CTX(ctx);
+  if (dtype == ITG_C64) return itg_axpby(ctx, ITG_F64, 2 * n, alpha, x, beta, y);
   ITG_CHECK_DTYPE(ctx, dtype);
   if (n <= 0) return ITG_OK;
   if (dtype == ITG_F64)
@@ -438,6 +451,11 @@ __global__ __launch_bounds__(256) void sum_parts_kernel(int n, const double* par
 static int dot_common(itg_ctx* ctx, int dtype, int64_t n, const void* x, const void* y,
                       bool sq, double* out) {
   ITG_CHECK_CTX(ctx);
+  if (dtype == ITG_C64) {
+    if (!out) return itg_fail(ctx, ITG_EARG, "null output");
+    if (sq) return dot_common(ctx, ITG_F64, 2 * n, x, nullptr, true, out);
+    return itg_c64_dot(ctx, n, x, y, out);
+  }
   ITG_CHECK_DTYPE(ctx, dtype);
   if (!out) return itg_fail(ctx, ITG_EARG, "null output");
   if (n <= 0) { *out = 0.0; return ITG_OK; }
@@ -623,9 +641,34 @@ static int permute_launch(itg_ctx* ctx, int rank, const int64_t* dimsA, const in
   return ITG_OK;
 }
 
+// complex storage as real storage with a leading re/im mode of dimension 2
+// that the permutation keeps in front
+static int c64_as_real(itg_ctx* ctx, int rank, const int64_t* dimsA, const int32_t* perm,
+                       int64_t* d2, int32_t* p2) {
+  if (rank < 0 || rank + 1 > ITG_MAX_RANK)
+    return itg_fail(ctx, ITG_EDIM, "complex permute: rank %d out of range", rank);
+  d2[0] = 2;
+  p2[0] = 0;
+  for (int q = 0; q < rank; ++q) {
+    if (perm[q] < 0 || perm[q] >= rank) return itg_fail(ctx, ITG_EDIM, "invalid permutation");
+    d2[q + 1] = dimsA[q];
+    p2[q + 1] = perm[q] + 1;
+  }
+  return ITG_OK;
+}
+
 extern "C" int itg_permute(itg_ctx* ctx, int dtype, int rank, const int64_t* dimsA,
                            const int32_t* perm, double alpha, const void* A, void* B) {
   ITG_CHECK_CTX(ctx);
+  if (dtype == ITG_C64) {
+    if (A == B) return itg_fail(ctx, ITG_EARG, "permute cannot run in place");
+    int64_t d2[ITG_MAX_RANK];
+    int32_t p2[ITG_MAX_RANK];
+    int rc = c64_as_real(ctx, rank, dimsA, perm, d2, p2);
+    if (rc) return rc;
+    return permute_launch<double>(ctx, rank + 1, d2, p2, alpha, (const double*)A, 0.0,
+                                  (double*)B, false);
+  }
   ITG_CHECK_DTYPE(ctx, dtype);
   if (A == B) return itg_fail(ctx, ITG_EARG, "permute cannot run in place");
   if (dtype == ITG_F64)
@@ -639,6 +682,13 @@ extern "C" int itg_add_permuted(itg_ctx* ctx, int dtype, int rank, const int64_t
                                 const int32_t* perm, double alpha, const void* A, double beta,
                                 void* B) {
   ITG_CHECK_CTX(ctx);
+  if (dtype == ITG_C64) {
+    int64_t d2[ITG_MAX_RANK];
+    int32_t p2[ITG_MAX_RANK];
+    int rc = c64_as_real(ctx, rank, dimsA, perm, d2, p2);
+    if (rc) return rc;
+    return itg_add_permuted(ctx, ITG_F64, rank + 1, d2, p2, alpha, A, beta, B);
+  }
   ITG_CHECK_DTYPE(ctx, dtype);
   if (A == B) {
     for (int q = 0; q < rank; ++q)

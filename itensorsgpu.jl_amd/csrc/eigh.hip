@@ -1311,7 +1311,8 @@ extern "C" int itg_eigh_trunc(itg_ctx* ctx, int dtype, int64_t n64, void* Ain, d
                               void* Uout, int64_t maxdim, double cutoff, int abs_cut, int rel_cut,
                               int64_t* kout, double* truncerr, double* docut) {
   ITG_CHECK_CTX(ctx);
-  ITG_CHECK_DTYPE(ctx, dtype);
+  const bool cplx = dtype == ITG_C64;   // Hermitian (heevd's route)
+  if (!cplx) ITG_CHECK_DTYPE(ctx, dtype);
   if (n64 < 0) return itg_fail(ctx, ITG_EDIM, "eigh: negative dimension");
   if (n64 > 8192) return itg_fail(ctx, ITG_EUNSUPPORTED, "eigh: n = %lld > 8192", (long long)n64);
   if (!kout) return itg_fail(ctx, ITG_EARG, "eigh: null k");
@@ -1331,7 +1332,7 @@ extern "C" int itg_eigh_trunc(itg_ctx* ctx, int dtype, int64_t n64, void* Ain, d
   // 113 ms against the one-stage reduction's 77 ms, its bulge chasing and
   // tall-panel QRs being latency-bound
   const char* two_env = getenv("ITG_EIGH_TWOSTAGE");
-  const bool two = n >= TWO_STAGE_MIN && two_env && two_env[0] == '1';
+  const bool two = !cplx && n >= TWO_STAGE_MIN && two_env && two_env[0] == '1';
   const size_t Tw = (size_t)(n + 63) / 64;   // tile rows of the tiled symv
   size_t bytes = (3 * nn + (f32 ? 2 * nn : 0)) * sizeof(double) + 3 * (size_t)n * NB * 8 +
                  (2 * (size_t)BTG * NB * n + (size_t)BTG * NB * BTG * NB) * 8 +
@@ -1339,7 +1340,8 @@ extern "C" int itg_eigh_trunc(itg_ctx* ctx, int dtype, int64_t n64, void* Ain, d
                  (size_t)n * 8 * 26 + 4 * (size_t)n * NB * 8 + 4 * NB * NB * 8 + 64 * 1024 +
                  (Tw * n + Tw * (Tw + 1) / 2 + 64) * 8 + 64 * 256 +
                  ((size_t)n / POST_ROWS + 1) * 8 +
-                 (two ? eig2::doubles_needed(n) * 8 + 256 : 0);
+                 (two ? eig2::doubles_needed(n) * 8 + 256 : 0) +
+                 (cplx ? (nn + itg_zhetrd_scratch_doubles(n) + 2 * (size_t)n) * 8 + 512 : 0);
   void* ws;
   rc = eig_workspace(ctx, bytes, &ws);
   if (rc) return rc;
@@ -1348,8 +1350,12 @@ extern "C" int itg_eigh_trunc(itg_ctx* ctx, int dtype, int64_t n64, void* Ain, d
   double* Qb = cv.take<double>(nn);
   // D&C merge vectors; afterwards the back-transform's compact-WY factors
   double* Vbuf = cv.take<double>(std::max(nn, (size_t)((n + NB - 1) / NB) * NB * NB));
-  double* A = f32 ? cv.take<double>(nn) : (double*)Ain;
-  double* U = f32 ? cv.take<double>(nn) : (double*)Uout;
+  // complex: A stays the caller's (complex) matrix, U a real n x k staging
+  // buffer for the tridiagonal's eigenvectors
+  double* A = f32 ? cv.take<double>(nn) : cplx ? nullptr : (double*)Ain;
+  double* U = (f32 || cplx) ? cv.take<double>(nn) : (double*)Uout;
+  double* zscr = cplx ? cv.take<double>(itg_zhetrd_scratch_doubles(n)) : nullptr;
+  double* ztau = cplx ? cv.take<double>(2 * (size_t)n) : nullptr;
   const size_t Tmax = (size_t)(n + TS - 1) / TS;
   double* VW = cv.take<double>(3 * (size_t)n * NB);
   double* Ypart = cv.take<double>(Tmax * n);
@@ -1383,7 +1389,8 @@ extern "C" int itg_eigh_trunc(itg_ctx* ctx, int dtype, int64_t n64, void* Ain, d
     if (rc) return rc;
   }
   // 1. tridiagonalise
-  if (two) rc = eig2::reduce(ctx, n, A, d, e, eb);
+  if (cplx) rc = itg_zhetrd(ctx, n, Ain, d, e, ztau, zscr);
+  else if (two) rc = eig2::reduce(ctx, n, A, d, e, eb);
   else rc = tridiagonalize(ctx, n, A, d, e, tau, VW, Ypart, ab, pnorm, pyv);
   if (rc) return rc;
   // 2. divide and conquer
@@ -1437,7 +1444,9 @@ extern "C" int itg_eigh_trunc(itg_ctx* ctx, int dtype, int64_t n64, void* Ain, d
   // 4. kept eigenvectors: gather, back-transform
   gather_cols_kernel<<<itg_blocks((int64_t)n * k, 256, 16384), 256, 0, ctx->stream>>>(n, (int)k, Q, perm, U);
   ITG_LAUNCH_CHECK(ctx);
-  if (two) {
+  if (cplx) {
+    rc = itg_zunmtr(ctx, n, (int)k, Ain, ztau, U, Uout);
+  } else if (two) {
     rc = eig2::apply_q2(ctx, n, (int)k, U, eb);
     if (rc) return rc;
     rc = back_transform_trd(ctx, n, (int)k, A, nullptr, U, eb.Tall, Ygrp, X2grp, Ggrp, eig2::SB,

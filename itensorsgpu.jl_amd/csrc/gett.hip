@@ -683,6 +683,8 @@ extern "C" int itg_contract(itg_ctx* ctx, int dtype, int mode, int rA, const int
                             const int32_t* lB, const void* B, int rC, const int64_t* dC,
                             const int32_t* lC, void* C, double alpha, double beta) {
   ITG_CHECK_CTX(ctx);
+  if (dtype == ITG_C64)
+    return itg_c64_contract(ctx, mode, rA, dA, lA, A, rB, dB, lB, B, rC, dC, lC, C, alpha, beta);
   ITG_CHECK_DTYPE(ctx, dtype);
   if (mode < 0 || mode > 2) return itg_fail(ctx, ITG_EARG, "unknown mode %d", mode);
   if (rA < 0 || rB < 0 || rC < 0 || rA > ITG_MAX_RANK || rB > ITG_MAX_RANK || rC > ITG_MAX_RANK)

@@ -55,6 +55,9 @@ struct itg_ctx {
   // instead of the grow-only workspace: the no-alloc entry points
   void* fixed_ws = nullptr;
   size_t fixed_ws_bytes = 0;
+  // grow-only device workspace of the complex contraction's expanded operand
+  void* cws = nullptr;
+  size_t cws_bytes = 0;
 };
 
 constexpr int kRedSlots = 16384;
@@ -89,7 +92,7 @@ int itg_fail(itg_ctx* ctx, int code, const char* fmt, ...);
       return itg_fail((ctx), ITG_EDTYPE, "unsupported dtype %d", (int)(dt));  \
   } while (0)
 
-static inline size_t itg_dtype_size(int dt) { return dt == ITG_F64 ? 8 : 4; }
+static inline size_t itg_dtype_size(int dt) { return dt == ITG_C64 ? 16 : dt == ITG_F64 ? 8 : 4; }
 
 // Grow-only workspace; the returned pointer is valid until the next call.
 int itg_workspace(itg_ctx* ctx, size_t bytes, void** out);
@@ -129,3 +132,22 @@ int itg_noalloc_setup(itg_ctx* ctx);
 int itg_heff2_apply_internal(itg_ctx* ctx, int dtype, int mode, const int64_t* dims,
                              const void* L, const void* R, const void* prep, const void* psi,
                              void* out, void* work);
+
+// ComplexF64 (complex.hip): pieces the generic entry points dispatch to
+int itg_c64_fill(itg_ctx* ctx, int64_t n, double value, void* x);
+int itg_c64_convert(itg_ctx* ctx, int64_t n, int sdt, const void* src, int ddt, void* dst);
+// out[0], out[1] = re, im of sum conj(x) y
+int itg_c64_dot(itg_ctx* ctx, int64_t n, const void* x, const void* y, double* out);
+int itg_c64_contract(itg_ctx* ctx, int mode, int rA, const int64_t* dA, const int32_t* lA,
+                     const void* A, int rB, const int64_t* dB, const int32_t* lB, const void* B,
+                     int rC, const int64_t* dC, const int32_t* lC, void* C, double alpha,
+                     double beta);
+// Hermitian A (n x n complex, full storage) -> real tridiagonal (d, e); the
+// reflectors stay in A's strict lower part, their factors in tau (n complex);
+// scratch: itg_zhetrd_scratch_doubles(n) doubles
+size_t itg_zhetrd_scratch_doubles(int n);
+int itg_zhetrd(itg_ctx* ctx, int n, void* A, double* d, double* e, void* tau, double* scratch);
+// U (n x k complex) = Q Z, Q the product of itg_zhetrd's reflectors, Z real
+// (n x k, leading dimension n)
+int itg_zunmtr(itg_ctx* ctx, int n, int k, const void* A, const void* tau, const double* Z,
+               void* U);

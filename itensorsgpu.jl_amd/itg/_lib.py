@@ -15,7 +15,7 @@ LIB_PATH = os.path.join(_HERE, "libitg.so")
 
 ITG_OK, ITG_EDIM, ITG_EDTYPE, ITG_EOOM, ITG_EHIP, ITG_EUNSUPPORTED, ITG_EARG, ITG_ECONV = \
     0, -1, -2, -3, -4, -5, -6, -7
-F64, F32 = 0, 1
+F64, F32, C64 = 0, 1, 2
 MODE_NATIVE, MODE_F32, MODE_BF16X3 = 0, 1, 2
 TRUNC_NDTENSORS, TRUNC_REFGPU = 0, 1
 MODES = {"fp64": MODE_NATIVE, "native": MODE_NATIVE, "fp32": MODE_F32, "bf16x3": MODE_BF16X3}
@@ -73,6 +73,10 @@ _SIGS = {
     "itg_axpby": (C.c_int, [_ctxp, C.c_int, C.c_int64, C.c_double, _vp, C.c_double, _vp]),
     "itg_precond_diag": (C.c_int, [_ctxp, C.c_int, C.c_int64, _vp, _vp, C.c_double, C.c_double,
                                    _vp]),
+    "itg_zscal": (C.c_int, [_ctxp, C.c_int64, C.c_double, C.c_double, _vp]),
+    "itg_zaxpby": (C.c_int, [_ctxp, C.c_int64, C.c_double, C.c_double, _vp, C.c_double,
+                             C.c_double, _vp]),
+    "itg_conj": (C.c_int, [_ctxp, C.c_int, C.c_int64, _vp]),
     "itg_dot": (C.c_int, [_ctxp, C.c_int, C.c_int64, _vp, _vp, _dp]),
     "itg_nrm2": (C.c_int, [_ctxp, C.c_int, C.c_int64, _vp, _dp]),
     "itg_contract": (C.c_int, [_ctxp, C.c_int, C.c_int,
@@ -239,6 +243,8 @@ def dtype_code(t):
         return F64
     if t.dtype == torch.float32:
         return F32
+    if t.dtype == torch.complex128:
+        return C64
     raise ArgumentError("unsupported dtype %s" % (t.dtype,))
 
 

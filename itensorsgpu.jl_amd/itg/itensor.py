@@ -112,10 +112,12 @@ class ITensor:
         return ITensor(inds, B)
 
     def __mul__(self, other):
-        if isinstance(other, (int, float)):
+        if isinstance(other, (int, float, complex, np.number)):
             self._need_device()
             out = ops.copy(self.data)
-            ops.scal(out, float(other))
+            if isinstance(other, (complex, np.complexfloating)) and not out.is_complex():
+                out = ops.convert(out, "complex128")
+            ops.scal(out, other)
             return ITensor(self.inds, out)
         return contract(self, other)
 
@@ -152,7 +154,8 @@ class ITensor:
     def scalar(self):
         if self.inds:
             raise _lib.DimensionMismatch("scalar of a tensor with indices")
-        return float(self.cpu().data[0])
+        v = self.cpu().data[0]
+        return complex(v) if np.iscomplexobj(v) else float(v)
 
     def prime(self, n=1, inds=None):
         sel = set(inds) if inds is not None else None
@@ -162,7 +165,24 @@ class ITensor:
         return ITensor([i.noprime() for i in self.inds], self.data)
 
     def dag(self):
-        return self   # real storage
+        """Complex conjugate (the reference's dag on Dense storage; index
+        arrows do not exist in this dense mirror). Real storage is shared."""
+        if not self.is_complex():
+            return self
+        if not self.ondevice:
+            return ITensor(self.inds, np.conj(self.data))
+        return ITensor(self.inds, ops.conj(ops.copy(self.data)))
+
+    def is_complex(self):
+        return bool(np.iscomplexobj(self.data)) if not self.ondevice else self.data.is_complex()
+
+    def complex(self):
+        """complex(T): ComplexF64 copy of real storage."""
+        if self.is_complex():
+            return self
+        if not self.ondevice:
+            return ITensor(self.inds, self.data.astype(np.complex128))
+        return ITensor(self.inds, ops.convert(self.data, "complex128"))
 
 
 # ---------------------------------------------------------------- factories
@@ -295,17 +315,19 @@ def factorize(A, Linds, ortho="left", which_decomp=None, maxdim=None, mindim=1, 
             return U, contract(S, V), spec
         return contract(U, S), V, spec
     if which_decomp == "eigen":
+        # rho = A dag(A') with the other side summed: rho(l, l') =
+        # sum_r A(l, r) conj(A(l', r)); rho = U D U^H, the other factor dag(U) A
         if ortho == "left":
             Ap = A.prime(1, Li)
-            rho = contract(A, Ap)           # (Li', Li) with the right indices summed
+            rho = contract(A, Ap.dag())     # (Li, Li') with the right indices summed
             D, U, spec = eigen(rho.noprime_pairs(Li), Li, [i.prime() for i in Li],
                                maxdim=maxdim, cutoff=cutoff, tags=tags)
-            return U, contract(U, A), spec
+            return U, contract(U.dag(), A), spec
         Ap = A.prime(1, Ri)
-        rho = contract(A, Ap)
+        rho = contract(A, Ap.dag())
         D, U, spec = eigen(rho.noprime_pairs(Ri), Ri, [i.prime() for i in Ri], maxdim=maxdim,
                            cutoff=cutoff, tags=tags)
-        return contract(A, U), U, spec
+        return contract(A, U.dag()), U, spec
     raise _lib.ArgumentError("factorize: unknown which_decomp %r" % (which_decomp,))
 
 
@@ -317,3 +339,22 @@ def _noprime_pairs(self, inds):
 
 
 ITensor.noprime_pairs = _noprime_pairs
+
+
+# ------------------------------------------------------------------ davidson
+def davidson(M, v, maxiter=2, errgoal=1e-14):
+    """davidson(M, v; maxiter) of ITensors' iterative solvers for a Hermitian
+    map M: ITensor -> ITensor with v's indices (test/test_cuiterativesolvers.jl:
+    ITensorMap), on real or complex storage. Returns (lambda, v)."""
+    from .mps import davidson as _dav
+    v._need_device()
+    inds = v.inds
+
+    def mv(x):
+        y = M(ITensor(inds, x))
+        if list(y.inds) != list(inds):
+            y = y.permute(*inds)
+        return y.data
+
+    lam, phi = _dav(mv, v.data, maxiter=maxiter, errgoal=errgoal)
+    return lam, ITensor(inds, phi)

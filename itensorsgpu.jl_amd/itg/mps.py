@@ -426,12 +426,13 @@ def davidson(matvec, x0, maxiter=2, miniter=1, errgoal=1e-14, northo=2, precond=
     nrm = ops.nrm2(x0)
     phi = ops.copy(x0)
     ops.scal(phi, 1.0 / nrm)
+    cplx = phi.is_complex()     # Hermitian problem on complex storage
     V = [phi]
     AV = [matvec(phi)]
-    lam = ops.dot(V[0], AV[0])
+    lam = ops.dot(V[0], AV[0]).real
     q = ops.copy(AV[0])
     ops.axpby(-lam, V[0], 1.0, q)
-    M = np.array([[lam]])
+    M = np.array([[lam]], dtype=complex if cplx else float)
     last = float("nan")
     maxiter = min(maxiter, phi.numel() - 1)
     nmv, nrev = 1, 0
@@ -463,13 +464,18 @@ def davidson(matvec, x0, maxiter=2, miniter=1, errgoal=1e-14, northo=2, precond=
             AV.append(matvec(t))
             nmv += 1
             m = len(V)
-            Mn = np.zeros((m, m))
+            Mn = np.zeros((m, m), dtype=M.dtype)
             Mn[: m - 1, : m - 1] = M
             for k in range(m):
-                Mn[k, m - 1] = Mn[m - 1, k] = ops.dot(V[k], AV[m - 1])
+                Mn[k, m - 1] = ops.dot(V[k], AV[m - 1])
+                Mn[m - 1, k] = np.conj(Mn[k, m - 1])
             ev, U = np.linalg.eigh(Mn)
             u = U[:, 0]
-            if u[0] < 0:
+            if cplx:
+                # phase so that the first coefficient is real and >= 0
+                if abs(u[0]) > 0:
+                    u = u * (np.conj(u[0]) / abs(u[0]))
+            elif u[0] < 0:
                 u = -u
             nphi = ops.copy(V[0])
             ops.scal(nphi, u[0])

@@ -48,10 +48,16 @@ def to_device(host_flat, dtype="float64"):
     return t
 
 
+_NP = {"float64": np.float64, "float32": np.float32, "complex128": np.complex128}
+
+
+def dtype_name(t):
+    return str(t.dtype).split(".")[-1]
+
+
 def to_host(t):
     """cpu(.) data movement: device buffer -> host numpy array (flat)."""
-    torch = _torch()
-    out = np.empty(t.numel(), dtype=np.float64 if t.dtype == torch.float64 else np.float32)
+    out = np.empty(t.numel(), dtype=_NP[dtype_name(t)])
     if out.size:
         c = ctx()
         c.check(lib().itg_memcpy_d2h(c.ptr, out.ctypes.data_as(C.c_void_p), ptr(t), out.nbytes))
@@ -79,18 +85,44 @@ def convert(t, dtype):
 
 
 # ------------------------------------------------------------------ BLAS-1
+def _is_cplx(a):
+    return isinstance(a, complex) or np.iscomplexobj(a)
+
+
 def scal(t, alpha):
+    """t *= alpha; a complex alpha needs complex storage."""
     c = ctx()
+    if _is_cplx(alpha):
+        if not t.is_complex():
+            raise _lib.ArgumentError("complex scale of real storage")
+        a = complex(alpha)
+        c.check(lib().itg_zscal(c.ptr, t.numel(), a.real, a.imag, ptr(t)))
+        return t
     c.check(lib().itg_scal(c.ptr, dtype_code(t), t.numel(), float(alpha), ptr(t)))
     return t
 
 
 def axpby(alpha, x, beta, y):
-    """y = alpha*x + beta*y (in place on y)."""
+    """y = alpha*x + beta*y (in place on y); complex coefficients need
+    complex storage."""
     c = ctx()
+    if _is_cplx(alpha) or _is_cplx(beta):
+        if not (x.is_complex() and y.is_complex()):
+            raise _lib.ArgumentError("complex coefficients on real storage")
+        a, b = complex(alpha), complex(beta)
+        c.check(lib().itg_zaxpby(c.ptr, y.numel(), a.real, a.imag, ptr(x), b.real, b.imag,
+                                 ptr(y)))
+        return y
     c.check(lib().itg_axpby(c.ptr, dtype_code(y), y.numel(), float(alpha), ptr(x), float(beta),
                             ptr(y)))
     return y
+
+
+def conj(t):
+    """t = conj(t) in place (no-op on real storage)."""
+    c = ctx()
+    c.check(lib().itg_conj(c.ptr, dtype_code(t), t.numel(), ptr(t)))
+    return t
 
 
 def precond_diag(q, diag, lam, floor=1e-8):
@@ -103,10 +135,13 @@ def precond_diag(q, diag, lam, floor=1e-8):
 
 
 def dot(x, y):
+    """sum conj(x) y (FP64-accumulated); complex for complex storage."""
     c = ctx()
-    out = C.c_double()
-    c.check(lib().itg_dot(c.ptr, dtype_code(x), x.numel(), ptr(x), ptr(y), C.byref(out)))
-    return out.value
+    out = (C.c_double * 2)()
+    c.check(lib().itg_dot(c.ptr, dtype_code(x), x.numel(), ptr(x), ptr(y), out))
+    if x.is_complex():
+        return complex(out[0], out[1])
+    return out[0]
 
 
 def nrm2(x):

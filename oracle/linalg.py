@@ -8,9 +8,11 @@ from .truncate import truncate
 def eigh_trunc(A, maxdim=None, cutoff=-1.0, absolute_cutoff=False,
                relative_cutoff=True, semantics="ndtensors"):
     """syevd -> reverse to descending -> truncate! (mindim not passed, as at
-    culinearalgebra.jl:92) -> keep leading columns. Returns
+    culinearalgebra.jl:92) -> keep leading columns; heevd (zheevd) for a
+    complex Hermitian A (culinearalgebra.jl:84-85). Returns
     (w_desc_kept, U_kept, truncerr, docut)."""
-    A = np.asarray(A, dtype=np.float64)
+    A = np.asarray(A)
+    A = A.astype(np.complex128 if np.iscomplexobj(A) else np.float64)
     n = A.shape[0]
     w, U = np.linalg.eigh(A, UPLO="L")
     w = w[::-1].copy()

@@ -51,3 +51,10 @@ def randn(n, seed, offset=0):
     th = 2.0 * np.pi * u2
     odd = (g & np.uint64(1)).astype(bool)
     return np.where(odd, r * np.sin(th), r * np.cos(th))
+
+
+def randn_complex(n, seed, offset=0):
+    """n complex Gaussians of unit variance (randn(ComplexF64)): element i is
+    (g[2(offset+i)] + i g[2(offset+i)+1]) / sqrt(2) over the real stream g."""
+    g = randn(2 * n, seed, 2 * offset) * 0.70710678118654752440
+    return g[0::2] + 1j * g[1::2]
