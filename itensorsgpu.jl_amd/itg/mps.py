@@ -39,7 +39,8 @@ def spin_ops(S):
     for i in range(1, d):
         Sp[i - 1, i] = math.sqrt(S * (S + 1) - m[i] * (m[i] + 1))
     Sm = Sp.T.copy()
-    return {"Sz": Sz, "S+": Sp, "S-": Sm, "Sx": 0.5 * (Sp + Sm), "Id": np.eye(d)}
+    return {"Sz": Sz, "S+": Sp, "S-": Sm, "Sx": 0.5 * (Sp + Sm), "Sy": -0.5j * (Sp - Sm),
+            "Id": np.eye(d)}
 
 
 class OpSum:
@@ -55,16 +56,24 @@ class OpSum:
         self.two = []   # (coef, opA, i, opB, j)
 
     def add(self, coef, op1, i, op2=None, j=None):
+        coef = complex(coef) if np.iscomplexobj(coef) and complex(coef).imag != 0 else float(
+            np.real(coef))
         if op2 is None:
-            self.one.append((float(coef), op1, int(i)))
+            self.one.append((coef, op1, int(i)))
         else:
             i, j = int(i), int(j)
             if i == j:
                 raise _lib.ArgumentError("two-site term on one site")
             if i > j:
                 op1, op2, i, j = op2, op1, j, i
-            self.two.append((float(coef), op1, i, op2, j))
+            self.two.append((coef, op1, i, op2, j))
         return self
+
+    def is_complex(self):
+        """Complex coefficients or an Sy operator: the MPO is ComplexF64."""
+        terms = [(c, o) for (c, o, _) in self.one] + \
+            [(c, a) for (c, a, _, _, _) in self.two] + [(c, b) for (c, _, _, b, _) in self.two]
+        return any(isinstance(c, complex) or o == "Sy" for c, o in terms)
 
     def to_mpo_arrays(self, N, S=0.5):
         ops_ = spin_ops(S)
@@ -81,7 +90,8 @@ class OpSum:
         for k in range(N):
             left = ["done"] + (chans[k - 1] if k > 0 else []) + ["start"] if k > 0 else ["start"]
             right = ["done"] + (chans[k] if k < N - 1 else []) + ["start"] if k < N - 1 else ["done"]
-            W = np.zeros((len(left), len(right), d, d))
+            W = np.zeros((len(left), len(right), d, d),
+                         dtype=np.complex128 if self.is_complex() else np.float64)
             li = {c: a for a, c in enumerate(left)}
             ri = {c: a for a, c in enumerate(right)}
             I = ops_["Id"]
@@ -143,17 +153,32 @@ def cylinder_bonds(Lx, Ly, snake=True):
 
 # ------------------------------------------------------------- containers
 class MPO:
-    """Device MPO: per site a flat buffer with dims (wl, wr, d, d)."""
+    """Device MPO: per site a flat buffer with dims (wl, wr, d, d); complex
+    arrays give a ComplexF64 MPO."""
 
-    def __init__(self, arrays, dtype="float64"):
+    def __init__(self, arrays, dtype=None):
+        if dtype is None:
+            dtype = "complex128" if any(np.iscomplexobj(W) for W in arrays) else "float64"
         self.dims = [tuple(int(x) for x in W.shape) for W in arrays]
         self.data = [ops.to_device(np.ravel(W, order="F"), dtype) for W in arrays]
 
     def __len__(self):
         return len(self.data)
 
+    def is_complex(self):
+        return self.data[0].is_complex()
+
+    def complex(self):
+        """The same operator on ComplexF64 storage."""
+        if self.is_complex():
+            return self
+        out = MPO.__new__(MPO)
+        out.dims = list(self.dims)
+        out.data = [ops.convert(t, "complex128") for t in self.data]
+        return out
+
     @staticmethod
-    def from_opsum(os_, N, S=0.5, dtype="float64"):
+    def from_opsum(os_, N, S=0.5, dtype=None):
         return MPO(os_.to_mpo_arrays(N, S), dtype)
 
 
@@ -171,6 +196,16 @@ class MPS:
 
     def maxlinkdim(self):
         return max(dm[2] for dm in self.dims)
+
+    def is_complex(self):
+        return self.data[0].is_complex()
+
+    def complex(self):
+        """complex(psi): the same state on ComplexF64 storage."""
+        if self.is_complex():
+            return self
+        return MPS([ops.convert(t, "complex128") for t in self.data], self.dims, self.llim,
+                   self.rlim)
 
     def to_host(self):
         return [np.reshape(ops.to_host(t), dm, order="F") for t, dm in zip(self.data, self.dims)]
@@ -292,18 +327,35 @@ def ones_env(dtype="float64"):
     return t, (1, 1, 1)
 
 
+def _bra(A):
+    """dag(A) on the storage: a conjugated copy for complex, A itself for real."""
+    return ops.conj(ops.copy(A)) if A.is_complex() else A
+
+
 def update_left(L, Ld, A, Ad, W, Wd, mode="native"):
-    """L_new(r', wr, r) = L(l', wl, l) A(l, s, r) W(wl, wr, s', s) A(l', s', r')."""
+    """L_new(r', wr, r) = L(l', wl, l) A(l, s, r) W(wl, wr, s', s) conj(A(l', s', r'))."""
     T1, d1 = ops.contract(L, Ld, [1, 2, 3], A, Ad, [3, 4, 5], [1, 2, 4, 5], mode=mode)
     T2, d2 = ops.contract(T1, d1, [1, 2, 4, 5], W, Wd, [2, 6, 7, 4], [1, 7, 5, 6])
-    return ops.contract(T2, d2, [1, 7, 5, 6], A, Ad, [1, 7, 8], [8, 6, 5], mode=mode)
+    return ops.contract(T2, d2, [1, 7, 5, 6], _bra(A), Ad, [1, 7, 8], [8, 6, 5], mode=mode)
 
 
 def update_right(R, Rd, A, Ad, W, Wd, mode="native"):
-    """R_new(l', wl, l) = R(r', wr, r) A(l, s, r) W(wl, wr, s', s) A(l', s', r')."""
+    """R_new(l', wl, l) = R(r', wr, r) A(l, s, r) W(wl, wr, s', s) conj(A(l', s', r'))."""
     T1, d1 = ops.contract(R, Rd, [1, 2, 3], A, Ad, [4, 5, 3], [1, 2, 4, 5], mode=mode)
     T2, d2 = ops.contract(T1, d1, [1, 2, 4, 5], W, Wd, [6, 2, 7, 5], [1, 7, 4, 6])
-    return ops.contract(T2, d2, [1, 7, 4, 6], A, Ad, [8, 7, 1], [8, 6, 4], mode=mode)
+    return ops.contract(T2, d2, [1, 7, 4, 6], _bra(A), Ad, [8, 7, 1], [8, 6, 4], mode=mode)
+
+
+def heff_apply_generic(L, Ld, W1, W1d, W2, W2d, R, Rd, x, xdims):
+    """The two-site Heff product as four FP64 contractions in the order
+    L.psi -> .W1 -> .W2 -> .R, on any storage (the complex path):
+    out(a', s1', s2', b') = L(a', wl, a) W1(wl, wm, s1', s1) W2(wm, wr, s2', s2)
+    R(b', wr, b) psi(a, s1, s2, b)."""
+    T1, d1 = ops.contract(L, Ld, [1, 2, 3], x, xdims, [3, 4, 5, 6], [1, 2, 4, 5, 6])
+    T2, d2 = ops.contract(T1, d1, [1, 2, 4, 5, 6], W1, W1d, [2, 7, 8, 4], [1, 7, 8, 5, 6])
+    T3, d3 = ops.contract(T2, d2, [1, 7, 8, 5, 6], W2, W2d, [7, 9, 10, 5], [1, 8, 10, 9, 6])
+    y, _ = ops.contract(T3, d3, [1, 8, 10, 9, 6], R, Rd, [11, 9, 6], [1, 8, 10, 11])
+    return y
 
 
 class ProjMPO:
@@ -334,20 +386,27 @@ class ProjMPO:
         return b
 
     def heff_work(self, dims):
+        if self.H.is_complex():
+            return None     # the complex product allocates its intermediates
         return self.buffer("work", _lib.lib().itg_heff2_work_elems(*dims), "float32")
 
     def build_right(self, psi, upto=2):
         N = len(psi)
-        self.R[N] = ones_env()
+        dt = ops.dtype_name(self.H.data[0])
+        self.R[N] = ones_env(dt)
         for j in range(N - 1, upto - 1, -1):
             R, Rd = self.R[j + 1]
             self.R[j] = update_right(R, Rd, psi.data[j], psi.dims[j], self.H.data[j],
                                      self.H.dims[j], self.env_mode)
-        self.L[0] = ones_env()
+        self.L[0] = ones_env(dt)
 
     def prepare(self, b, xdims):
         """Fused W1.W2 operator (and, in bf16x3, the split L and R) for the
-        Krylov loop at bond b; product() uses it until the next prepare."""
+        Krylov loop at bond b; product() uses it until the next prepare.
+        Complex operators have no prepared form."""
+        if self.H.is_complex():
+            self._prep = None
+            return
         L, Ld = self.L[b]
         R, Rd = self.R[b + 2]
         wl, wm = self.H.dims[b][0], self.H.dims[b][1]
@@ -394,7 +453,12 @@ class ProjMPO:
         dims = (chiL, wl, wm, wr, d, chiR)
         ev = self.timer.start() if self.timer is not None else None
         p = self._prep
-        if (p is not None and p[0] == b and p[1] == self.mode and p[2] == dims and p[4] is L
+        if self.H.is_complex():
+            if self.mode != "native":
+                raise _lib.ArgumentError("complex Heff runs in FP64 (native) mode only")
+            y = heff_apply_generic(L, Ld, self.H.data[b], self.H.dims[b], self.H.data[b + 1],
+                                   self.H.dims[b + 1], R, Rd, x, xdims)
+        elif (p is not None and p[0] == b and p[1] == self.mode and p[2] == dims and p[4] is L
                 and p[5] is R):
             y = ops.heff2_apply_prepared(L, R, p[3], x, *dims, mode=self.mode, out=out, work=work)
         else:
@@ -405,6 +469,8 @@ class ProjMPO:
         return y
 
     def diag(self, b, xdims):
+        if self.H.is_complex():
+            raise _lib.ArgumentError("the Heff diagonal is built for real operators only")
         L, Ld = self.L[b]
         R, Rd = self.R[b + 2]
         wl, wm = self.H.dims[b][0], self.H.dims[b][1]
@@ -535,8 +601,12 @@ def noiseterm(PH, b, phi, pdims, ortho, mode="native"):
                              mode=mode)
         nt, dn = ops.contract(T, dT, [1, 2, 4, 5, 6], W, Wd, [2, 7, 8, 4], [1, 8, 7, 5, 6])
         m, K = chiL * d1, Wd[1] * d2 * chiR
-        drho = ops.empty(m * m, dt)
-        ops.gemm("N", "T", m, m, K, 1.0, nt, m, nt, m, 0.0, drho, m, mode=mode)
+        if nt.is_complex():
+            # drho(i, i') = sum_K nt(i, K) conj(nt(i', K))
+            drho, _ = ops.contract(nt, (m, K), [1, 2], _bra(nt), (m, K), [3, 2], [1, 3])
+        else:
+            drho = ops.empty(m * m, dt)
+            ops.gemm("N", "T", m, m, K, 1.0, nt, m, nt, m, 0.0, drho, m, mode=mode)
     else:
         R, Rd = PH.R[b + 2]
         W, Wd = PH.H.data[b + 1], PH.H.dims[b + 1]
@@ -544,14 +614,45 @@ def noiseterm(PH, b, phi, pdims, ortho, mode="native"):
                              mode=mode)
         nt, dn = ops.contract(T, dT, [1, 2, 3, 5, 6], W, Wd, [7, 6, 8, 3], [1, 2, 7, 8, 5])
         n, K = d2 * chiR, chiL * d1 * Wd[0]
-        drho = ops.empty(n * n, dt)
-        ops.gemm("T", "N", n, n, K, 1.0, nt, K, nt, K, 0.0, drho, n, mode=mode)
+        if nt.is_complex():
+            # drho(j, j') = sum_K conj(nt(K, j)) nt(K, j')
+            drho, _ = ops.contract(_bra(nt), (K, n), [1, 2], nt, (K, n), [1, 3], [2, 3])
+        else:
+            drho = ops.empty(n * n, dt)
+            ops.gemm("T", "N", n, n, K, 1.0, nt, K, nt, K, 0.0, drho, n, mode=mode)
         m = n
     # (X + X^T)/2: the split-BF16 product sums its three partial products
     # in a different order for (i, j) and (j, i), so only this makes the
-    # term exactly symmetric in every mode
+    # term exactly symmetric in every mode ((X + X^H)/2: exactly Hermitian)
     sym, _ = ops.permute(drho, (m, m), (1, 0), alpha=0.5)
+    ops.conj(sym)
     return ops.add_permuted(drho, (m, m), (0, 1), 0.5, sym, 1.0)
+
+
+def _replacebond_complex(phi, m, n, ortho, maxdim, cutoff, drho, noise):
+    """Complex phi (m x n): left, rho = phi phi^H = U D U^H, A = U, B = U^H phi;
+    right, rho = phi^H phi = V D V^H, B = V^H, A = phi V (B, A normalised as
+    the real route does)."""
+    cut = 0.0 if cutoff is None else cutoff
+    pb = _bra(phi)
+    if ortho == "left":
+        rho, _ = ops.contract(phi, (m, n), [1, 2], pb, (m, n), [3, 2], [1, 3])
+        if drho is not None:
+            ops.axpby(noise, drho, 1.0, rho)
+        w, U, k, terr, _ = ops.eigh_trunc(rho, m, maxdim=maxdim, cutoff=cut)
+        A = U[: m * k].clone()
+        B, _ = ops.contract(_bra(A), (m, k), [1, 2], phi, (m, n), [1, 3], [2, 3])
+        ops.scal(B, 1.0 / ops.nrm2(B))
+    else:
+        rho, _ = ops.contract(pb, (m, n), [1, 2], phi, (m, n), [1, 3], [2, 3])
+        if drho is not None:
+            ops.axpby(noise, drho, 1.0, rho)
+        w, U, k, terr, _ = ops.eigh_trunc(rho, n, maxdim=maxdim, cutoff=cut)
+        V = U[: n * k].clone()
+        B, _ = ops.permute(_bra(V), (n, k), (1, 0))
+        A, _ = ops.contract(phi, (m, n), [1, 2], V, (n, k), [2, 3], [1, 3])
+        ops.scal(A, 1.0 / ops.nrm2(A))
+    return A, B, k, terr
 
 
 def replacebond(psi, b, phi, pdims, ortho, maxdim, mindim, cutoff, drho=None, noise=0.0):
@@ -563,7 +664,11 @@ def replacebond(psi, b, phi, pdims, ortho, maxdim, mindim, cutoff, drho=None, no
     m, n = l * d1, d2 * r
     use_eigen = (cutoff is not None and cutoff >= 1e-12) or drho is not None
     dt = str(phi.dtype).split(".")[-1]
-    if use_eigen:
+    if phi.is_complex():
+        # the density-matrix route on ComplexF64 (heevd); there is no
+        # complex SVD, so it is taken at every cutoff
+        A, B, k, terr = _replacebond_complex(phi, m, n, ortho, maxdim, cutoff, drho, noise)
+    elif use_eigen:
         if ortho == "left":
             rho = ops.empty(m * m, dt)
             ops.gemm("N", "T", m, m, n, 1.0, phi, m, phi, m, 0.0, rho, m)
@@ -664,6 +769,16 @@ def dmrg(H, psi, sweeps, mode="native", krylovdim=3, precond=None, errgoal=1e-14
     the records of the sweeps before it."""
     N = len(psi)
     history = list(history or [])
+    # ComplexF64: operator and state on complex storage, FP64 Heff as
+    # contractions, the host Davidson (complex inner products) and the
+    # Hermitian density-matrix route
+    if psi.is_complex() or H.is_complex():
+        if mode != "native":
+            raise _lib.ArgumentError("complex dmrg runs in FP64 (native) mode only")
+        H = H.complex()
+        if not psi.is_complex():
+            psi.data = [ops.convert(t, "complex128") for t in psi.data]
+        native = False
     # environments accumulate along the chain (L_b is b successive updates),
     # so their errors add up: they stay FP64 unless asked otherwise
     env_mode = "native" if env_mode is None else env_mode
@@ -786,12 +901,18 @@ def dmrg_resume(H, path, sweeps, **kwargs):
 
 
 def expect_energy(H, psi):
-    """<psi|H|psi>/<psi|psi> by a full left-to-right transfer on the device."""
-    L, Ld = ones_env()
-    Nn, Nd = ones_env()
+    """<psi|H|psi>/<psi|psi> by a full left-to-right transfer on the device
+    (real part: H is Hermitian)."""
+    cplx = psi.is_complex() or H.is_complex()
+    if cplx:
+        psi, H = psi.complex(), H.complex()
+    dt = "complex128" if cplx else "float64"
+    L, Ld = ones_env(dt)
+    Nn, Nd = ones_env(dt)
     for j in range(len(psi)):
         L, Ld = update_left(L, Ld, psi.data[j], psi.dims[j], H.data[j], H.dims[j])
-        # norm transfer: N(r', r) = N(l', l) A(l, s, r) A(l', s, r')
+        # norm transfer: N(r', r) = N(l', l) A(l, s, r) conj(A(l', s, r'))
         T, td = ops.contract(Nn, Nd, [1, 2, 3], psi.data[j], psi.dims[j], [3, 4, 5], [1, 2, 4, 5])
-        Nn, Nd = ops.contract(T, td, [1, 2, 4, 5], psi.data[j], psi.dims[j], [1, 4, 6], [6, 2, 5])
-    return ops.to_host(L)[0] / ops.to_host(Nn)[0]
+        Nn, Nd = ops.contract(T, td, [1, 2, 4, 5], _bra(psi.data[j]), psi.dims[j], [1, 4, 6],
+                              [6, 2, 5])
+    return float(np.real(ops.to_host(L)[0] / ops.to_host(Nn)[0]))

@@ -65,3 +65,36 @@ def test_long_range_terms():
 
     ref = 0.7 * op(Sz, 0) @ op(Sz, 4) - 0.3 * op(Sp, 1) @ op(Sm, 3) + 0.2 * op(0.5 * (Sp + Sm), 2)
     np.testing.assert_allclose(H, ref, atol=1e-14)
+
+
+def test_complex_dm_term_against_kronecker_products():
+    """A complex OpSum (a Dzyaloshinskii-Moriya term written with S+/S- and
+    imaginary coefficients, and the same term written with Sx/Sy) compiles to
+    a ComplexF64 MPO whose matrix is the Kronecker-product Hamiltonian
+    D sum_j (Sx_j Sy_j+1 - Sy_j Sx_j+1), Hermitian and not real."""
+    from itg.mps import OpSum
+    N, D = 4, 0.7
+    sx = np.array([[0, 0.5], [0.5, 0]])
+    sy = np.array([[0, -0.5j], [0.5j, 0]])
+    I2 = np.eye(2)
+
+    def site_op(op, j):
+        out = np.eye(1)
+        for k in range(N):
+            out = np.kron(out, op if k == j else I2)
+        return out
+
+    Hk = sum(D * (site_op(sx, j) @ site_op(sy, j + 1) - site_op(sy, j) @ site_op(sx, j + 1))
+             for j in range(N - 1))
+    a, b = OpSum(), OpSum()
+    for j in range(N - 1):
+        a.add(0.5j * D, "S+", j, "S-", j + 1)
+        a.add(-0.5j * D, "S-", j, "S+", j + 1)
+        b.add(D, "Sx", j, "Sy", j + 1)
+        b.add(-D, "Sy", j, "Sx", j + 1)
+    for os_ in (a, b):
+        Ws = os_.to_mpo_arrays(N)
+        assert all(W.dtype == np.complex128 for W in Ws)
+        H = models.mpo_to_matrix(Ws)
+        np.testing.assert_allclose(H, Hk, atol=1e-14)
+    assert np.abs(Hk - Hk.conj().T).max() < 1e-15 and np.abs(Hk.imag).max() > 0.1
