@@ -41,6 +41,7 @@ end
 
 dtype(::Type{Float64}) = Cint(0)
 dtype(::Type{Float32}) = Cint(1)
+dtype(::Type{ComplexF64}) = Cint(2)   # interleaved (re, im), Julia's layout
 
 # ------------------------------------------------------------- device vector
 mutable struct RocVector{T} <: AbstractVector{T}
@@ -125,7 +126,8 @@ const SEED = Ref{UInt64}(0)
 const OFFSET = Ref{UInt64}(0)
 seed!(s::Integer) = (SEED[] = UInt64(s); OFFSET[] = 0; nothing)
 
-function randn!(T::RocDenseTensor{ElT}) where {ElT<:Union{Float32,Float64}}
+# ComplexF64: unit-variance complex Gaussians, (g[2i] + i g[2i+1]) / sqrt(2)
+function randn!(T::RocDenseTensor{ElT}) where {ElT<:Union{Float32,Float64,ComplexF64}}
     n = dim(T)
     check(ccall((:itg_randn, libitg), Cint, (Ptr{Cvoid}, Cint, Int64, UInt64, UInt64, Ptr{ElT}),
                 ctx(), dtype(ElT), n, SEED[], OFFSET[], pointer(data(store(T)))))
@@ -137,8 +139,11 @@ randn!(T::ITensor) = (randn!(tensor(T)); T)
 rocITensor(::Type{T}, inds::Indices) where {T<:Number} =
     ITensor(Dense(zeros_roc(T, dim(inds))), inds)
 rocITensor(::Type{T}, inds::Index...) where {T<:Number} = rocITensor(T, IndexSet(inds...))
-randomRocITensor(::Type{S}, inds::Indices) where {S<:Real} = randn!(rocITensor(S, inds))
-randomRocITensor(::Type{S}, inds::Index...) where {S<:Real} = randomRocITensor(S, IndexSet(inds...))
+# real and ComplexF64 element types (randomCuITensor, src/cuitensor.jl:35-49)
+randomRocITensor(::Type{S}, inds::Indices) where {S<:Union{Real,ComplexF64}} =
+    randn!(rocITensor(S, inds))
+randomRocITensor(::Type{S}, inds::Index...) where {S<:Union{Real,ComplexF64}} =
+    randomRocITensor(S, IndexSet(inds...))
 randomRocITensor(inds::Indices) = randomRocITensor(Float64, inds)
 randomRocITensor(inds::Index...) = randomRocITensor(Float64, IndexSet(inds...))
 
@@ -223,10 +228,28 @@ function *(D::RocDense{ElT}, x::Number) where {ElT}
     v = RocVector{ElT}(undef, length(data(D)))
     check(ccall((:itg_memcpy_d2d, libitg), Cint, (Ptr{Cvoid}, Ptr{ElT}, Ptr{ElT}, Csize_t),
                 ctx(), v.ptr, pointer(data(D)), length(v) * sizeof(ElT)))
-    check(ccall((:itg_scal, libitg), Cint, (Ptr{Cvoid}, Cint, Int64, Cdouble, Ptr{ElT}),
-                ctx(), dtype(ElT), length(v), x, v.ptr))
+    if x isa Complex
+        ElT === ComplexF64 || throw(ArgumentError("complex scale of real storage"))
+        check(ccall((:itg_zscal, libitg), Cint, (Ptr{Cvoid}, Int64, Cdouble, Cdouble, Ptr{ElT}),
+                    ctx(), length(v), real(x), imag(x), v.ptr))
+    else
+        check(ccall((:itg_scal, libitg), Cint, (Ptr{Cvoid}, Cint, Int64, Cdouble, Ptr{ElT}),
+                    ctx(), dtype(ElT), length(v), x, v.ptr))
+    end
     return Dense(v)
 end
+
+# conj of complex storage (what dag does to the data); real storage is shared
+function Base.conj(D::RocDense{ComplexF64})
+    v = RocVector{ComplexF64}(undef, length(data(D)))
+    check(ccall((:itg_memcpy_d2d, libitg), Cint,
+                (Ptr{Cvoid}, Ptr{ComplexF64}, Ptr{ComplexF64}, Csize_t),
+                ctx(), v.ptr, pointer(data(D)), length(v) * sizeof(ComplexF64)))
+    check(ccall((:itg_conj, libitg), Cint, (Ptr{Cvoid}, Cint, Int64, Ptr{ComplexF64}),
+                ctx(), dtype(ComplexF64), length(v), v.ptr))
+    return Dense(v)
+end
+Base.conj(D::RocDense{<:Real}) = D
 /(A::RocDenseTensor, x::Number) = A * inv(x)
 
 function norm(T::RocDenseTensor{ElT}) where {ElT}
