@@ -76,6 +76,7 @@ extern "C" int itg_destroy(itg_ctx* ctx) {
   if (ctx->eig_ws) hipFree(ctx->eig_ws);
   if (ctx->mlp_ws) hipFree(ctx->mlp_ws);
   if (ctx->cws) hipFree(ctx->cws);
+  if (ctx->svd_ws) hipFree(ctx->svd_ws);
   if (ctx->red) hipFree(ctx->red);
   if (ctx->red_host) hipHostFree(ctx->red_host);
   hipStreamDestroy(ctx->own_stream);
@@ -724,6 +725,118 @@ extern "C" int itg_truncate(itg_ctx* ctx, const double* P, int64_t n, int64_t ma
 // staged in LDS (up to kTruncLds of them) so its scans do not wait on HBM.
 constexpr int64_t kTruncLds = 8192;
 
+// lane j's double, broadcast to the wave (j uniform)
+__device__ __forceinline__ double wave_lane(double v, int j) {
+  int lo = __builtin_amdgcn_readlane(__double2loint(v), j);
+  int hi = __builtin_amdgcn_readlane(__double2hiint(v), j);
+  return __hiloint2double(hi, lo);
+}
+
+// The NDTensors decision of itg_truncate_core for one wave: the weights are
+// read 64 at a time into registers (the next 64 in flight while the current
+// ones are used) and every sum is taken element by element in the core's
+// order through lane broadcasts, so (n, truncerr, docut) are bit-identical
+// to the host's while the serial chain runs on register operands.
+__device__ void truncate_ndtensors_wave(const double* P, int64_t m, int64_t maxdim,
+                                        int64_t mindim, double cutoff, int abs_cut, int rel_cut,
+                                        double* out) {
+  const int lane = threadIdx.x & 63;
+  if (maxdim <= 0 || maxdim > m) maxdim = m;
+  if (mindim < 1) mindim = 1;
+  if (cutoff < 0.0) cutoff = 0.0;
+  double rn = 1.0, rte = 0.0, rdc = 0.0;
+  const double p0 = P[0];
+  if (!(p0 <= 0.0) && m == 1) rdc = p0 / 2;
+  if (!(p0 <= 0.0) && m > 1) {
+    // lastpos = 1 + the last index whose weight is not negative
+    int64_t lp = 0;
+    for (int64_t i = lane; i < m; i += 64)
+      if (!(P[i] < 0.0)) lp = i + 1;
+    for (int o = 32; o > 0; o >>= 1) {
+      int64_t t = __shfl_xor(lp, o, 64);
+      lp = t > lp ? t : lp;
+    }
+    auto W = [&](int64_t i) { return (i >= 0 && i < lp && i < m) ? P[i] : 0.0; };
+    int64_t n = m;
+    double te = 0.0;
+    // from the tail: chunk c holds W(hi - 63 .. hi), lane l = W(hi - l)
+    int64_t hi = m - 1;
+    double cur = W(hi - lane), nxt = W(hi - 64 - lane);
+    int j = 0;   // next lane of `cur` to use
+    auto advance = [&]() {
+      if (++j == 64) {
+        j = 0;
+        hi -= 64;
+        cur = nxt;
+        nxt = W(hi - 64 - lane);
+      }
+    };
+    // whole chunks of the maxdim tail in one unrolled chain (constant lanes),
+    // the rest element by element
+    while (n - maxdim >= 64) {
+#pragma unroll
+      for (int l = 0; l < 64; ++l) te += wave_lane(cur, l);
+      n -= 64;
+      hi -= 64;
+      cur = nxt;
+      nxt = W(hi - 64 - lane);
+    }
+    while (n > maxdim) {
+      te += wave_lane(cur, j);
+      --n;
+      advance();
+    }
+    if (abs_cut) {
+      while (n > mindim) {
+        double w = wave_lane(cur, j);
+        if (!(w <= cutoff)) break;
+        te += w;
+        --n;
+        advance();
+      }
+    } else {
+      double scale = 1.0;
+      if (rel_cut) {
+        scale = 0.0;
+        double a = W(lane), b = W(64 + lane);
+        int64_t base = 0;
+        for (; base + 64 <= m; base += 64) {
+#pragma unroll
+          for (int l = 0; l < 64; ++l) scale += wave_lane(a, l);
+          a = b;
+          b = W(base + 128 + lane);
+        }
+        for (int l = 0; l < (int)(m - base); ++l) scale += wave_lane(a, l);
+        if (scale == 0.0) scale = 1.0;
+      }
+      const double lim = cutoff * scale;
+      while (n > mindim) {
+        double w = wave_lane(cur, j);
+        if (!(te + w <= lim)) break;
+        te += w;
+        --n;
+        advance();
+      }
+      te /= scale;
+    }
+    if (n < 1) n = 1;
+    rn = (double)n;
+    rte = te;
+    if (n < m) {
+      double a = W(n - 1), b = W(n);
+      rdc = (a + b) / 2.0;
+      double diff = a - b;
+      if (diff < 0) diff = -diff;
+      if (diff < 1e-3 * a) rdc += 1e-3 * a;
+    }
+  }
+  if (lane == 0) {
+    out[0] = rn;
+    out[1] = rte;
+    out[2] = rdc;
+  }
+}
+
 __global__ void truncate_kernel(const double* P, int64_t n, int64_t maxdim, int64_t mindim,
                                 double cutoff, int abs_cut, int rel_cut, int semantics,
                                 double* out) {
@@ -732,6 +845,11 @@ __global__ void truncate_kernel(const double* P, int64_t n, int64_t maxdim, int6
   if (staged) {
     for (int64_t i = threadIdx.x; i < n; i += blockDim.x) Ps[i] = P[i];
     __syncthreads();
+  }
+  if (semantics == ITG_TRUNC_NDTENSORS) {
+    if (threadIdx.x < 64)
+      truncate_ndtensors_wave(staged ? Ps : P, n, maxdim, mindim, cutoff, abs_cut, rel_cut, out);
+    return;
   }
   if (threadIdx.x == 0 && blockIdx.x == 0) {
     itg_trunc_result r = itg_truncate_core(staged ? Ps : P, n, maxdim, mindim, cutoff, abs_cut,

@@ -1697,6 +1697,10 @@ extern "C" int itg_qr(itg_ctx* ctx, int dtype, int64_t m64, int64_t n64, const v
   (void)f32;
 }
 
+// smallest column count at which the Jacobi SVD is preconditioned by the
+// Gram matrix's eigenvectors
+constexpr int SVD_PRECOND_MIN = 64;
+
 extern "C" int itg_svd_trunc(itg_ctx* ctx, int dtype, int64_t m64, int64_t n64, void* Ain, void* Uo,
                              double* So, void* Vo, int64_t maxdim, int64_t mindim, double cutoff,
                              int abs_cut, int rel_cut, int64_t* kout, double* truncerr,
@@ -1719,11 +1723,26 @@ extern "C" int itg_svd_trunc(itg_ctx* ctx, int dtype, int64_t m64, int64_t n64, 
   bool tr = m < n;                 // work on A^T so that rows >= cols
   int rows = tr ? n : m, cols = tr ? m : n;
   int N = cols + (cols & 1);
+  // preconditioning (below) runs the eigensolver, which owns eig_ws: the
+  // SVD's buffers live in their own grow-only workspace
+  const bool precond = cols >= SVD_PRECOND_MIN;
   size_t bytes = ((size_t)rows * cols * 2 + (size_t)cols * cols + (size_t)rows * cols +
-                  (size_t)cols * cols + 8 * (size_t)N + 64) * sizeof(double) + 16 * 256;
-  void* ws;
-  rc = eig_workspace(ctx, bytes, &ws);
-  if (rc) return rc;
+                  (size_t)cols * cols + 8 * (size_t)N + 64) * sizeof(double) + 16 * 256 +
+                 (precond ? ((size_t)rows * cols + (size_t)cols * cols + cols) * 8 + 3 * 256 : 0);
+  if (bytes > ctx->svd_ws_bytes) {
+    if (ctx->svd_ws) {
+      ITG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+      ITG_HIP(ctx, hipFree(ctx->svd_ws));
+      ctx->svd_ws = nullptr;
+      ctx->svd_ws_bytes = 0;
+    }
+    if (hipMalloc(&ctx->svd_ws, bytes) != hipSuccess) {
+      hipGetLastError();
+      return itg_fail(ctx, ITG_EOOM, "svd workspace of %zu bytes failed", bytes);
+    }
+    ctx->svd_ws_bytes = bytes;
+  }
+  void* ws = ctx->svd_ws;
   Carver cv{(char*)ws};
   double* A64 = cv.take<double>((size_t)m * n);
   double* W = cv.take<double>((size_t)rows * cols);
@@ -1740,8 +1759,31 @@ extern "C" int itg_svd_trunc(itg_ctx* ctx, int dtype, int64_t m64, int64_t n64, 
   if (rc) return rc;
   if (tr) transpose_kernel<<<itg_blocks((int64_t)m * n, 256, 8192), 256, 0, ctx->stream>>>(m, n, A64, W);
   else ITG_HIP(ctx, hipMemcpyAsync(W, A64, (size_t)m * n * 8, hipMemcpyDeviceToDevice, ctx->stream));
-  qr_init_q_kernel<<<itg_blocks((int64_t)cols * cols, 256, 8192), 256, 0, ctx->stream>>>(cols, cols, V);
-  ITG_LAUNCH_CHECK(ctx);
+  if (precond) {
+    // Jacobi preconditioned by the Gram matrix's eigenvectors: W <- W V0
+    // with W^T W = V0 diag(w) V0^T, V <- V0. The columns of W V0 are
+    // orthogonal up to eps ||A||^2 / (s_i s_j), so the one-sided sweeps
+    // below (which keep their relative accuracy) converge in a few sweeps
+    double* G = cv.take<double>((size_t)cols * cols);
+    double* W2 = cv.take<double>((size_t)rows * cols);
+    double* wG = cv.take<double>(cols);
+    if (cv.off > bytes) return itg_fail(ctx, ITG_EOOM, "svd: workspace layout overflow");
+    // (one operand buffer: the exactly symmetric product)
+    rc = itg_gemm(ctx, ITG_F64, ITG_MODE_NATIVE, 'T', 'N', cols, cols, rows, 1.0, W, rows, W, rows,
+                  0.0, G, cols);
+    if (rc) return rc;
+    int64_t kg = 0;
+    double teg = 0.0, dcg = 0.0;
+    rc = itg_eigh_trunc(ctx, ITG_F64, cols, G, wG, V, 0, -1.0, 0, 1, &kg, &teg, &dcg);
+    if (rc) return rc;
+    rc = itg_gemm_internal(ctx, ITG_F64, ITG_MODE_NATIVE, 'N', 'N', rows, cols, cols, 1.0, W, rows,
+                           V, cols, 0.0, W2, rows);
+    if (rc) return rc;
+    W = W2;
+  } else {
+    qr_init_q_kernel<<<itg_blocks((int64_t)cols * cols, 256, 8192), 256, 0, ctx->stream>>>(cols, cols, V);
+    ITG_LAUNCH_CHECK(ctx);
+  }
   double tol = 4.0 * ITG_DC_EPS * std::sqrt((double)rows);
   double fro = 0.0;
   rc = itg_nrm2(ctx, ITG_F64, (int64_t)m * n, A64, &fro);

@@ -55,6 +55,10 @@ struct itg_ctx {
   // instead of the grow-only workspace: the no-alloc entry points
   void* fixed_ws = nullptr;
   size_t fixed_ws_bytes = 0;
+  // grow-only device workspace of the SVD (apart from eig_ws: the SVD calls
+  // the eigensolver)
+  void* svd_ws = nullptr;
+  size_t svd_ws_bytes = 0;
   // grow-only device workspace of the complex contraction's expanded operand
   void* cws = nullptr;
   size_t cws_bytes = 0;
