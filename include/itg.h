@@ -183,6 +183,34 @@ int itg_gemm(itg_ctx* ctx, int dtype, int mode, char transA, char transB,
              int64_t lda, const void* B, int64_t ldb, double beta, void* C,
              int64_t ldc);
 
+/* ---------------------------------------------------------- Diag storage */
+/* C = alpha * sum_{contracted} D * B + beta * C for a Diag tensor D of rank
+ * rD (dims dD, labels lD): D(i1..irD) = ddata[k] when every index equals k
+ * (nd = min(dD) entries), else 0; with ddata NULL, `uniform` (one value,
+ * two doubles re/im for ITG_C64) is on the whole diagonal (UniformDiag).
+ * Labels as in itg_contract (no batch modes); a label of D kept in C takes
+ * the diagonal position from the output coordinates, and when every label of
+ * D is contracted the diagonal is summed. Replaces the Diag x Dense
+ * contract! methods of src/tensor/cudiag.jl:118-162 (the uniform one, which
+ * the reference runs as A.*B ignoring index order (test_cudiag.jl:49,95
+ * @test_broken), follows the labels here). */
+int itg_diag_contract(itg_ctx* ctx, int dtype, int rD, const int64_t* dD,
+                      const int32_t* lD, const void* ddata, int64_t nd,
+                      const double* uniform, int rB, const int64_t* dB,
+                      const int32_t* lB, const void* B, int rC, const int64_t* dC,
+                      const int32_t* lC, void* C, double alpha, double beta);
+/* z = x .* y element-wise: Diag x Diag storage products (cudiag.jl:118-140,
+ * `copyto!(Cstore, Astore.*Bstore)`). */
+int itg_hadamard(itg_ctx* ctx, int dtype, int64_t n, const void* x,
+                 const void* y, void* z);
+/* Copy S (rank `rank`, dims dimsS, column-major) into the block of D (dims
+ * dimsD) that starts at `offsets`; D's other entries are left as they are.
+ * The direct sum of link spaces in MPS/MPO `add` (ITensors' MPS +,
+ * test/test_cumpo.jl "add"). */
+int itg_embed(itg_ctx* ctx, int dtype, int rank, const int64_t* dimsS,
+              const void* S, const int64_t* dimsD, const int64_t* offsets,
+              void* D);
+
 /* ------------------------------------------------------------- truncation */
 /* P: host array of n non-increasing weights (squared singular values or
  * density-matrix eigenvalues). Returns the kept count, the truncation error

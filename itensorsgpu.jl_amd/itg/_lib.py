@@ -84,6 +84,11 @@ _SIGS = {
                                C.c_int, _i64p, _i32p, _vp,
                                C.c_int, _i64p, _i32p, _vp,
                                C.c_double, C.c_double]),
+    "itg_diag_contract": (C.c_int, [_ctxp, C.c_int, C.c_int, _i64p, _i32p, _vp, C.c_int64, _dp,
+                                    C.c_int, _i64p, _i32p, _vp, C.c_int, _i64p, _i32p, _vp,
+                                    C.c_double, C.c_double]),
+    "itg_hadamard": (C.c_int, [_ctxp, C.c_int, C.c_int64, _vp, _vp, _vp]),
+    "itg_embed": (C.c_int, [_ctxp, C.c_int, C.c_int, _i64p, _vp, _i64p, _i64p, _vp]),
     "itg_gemm": (C.c_int, [_ctxp, C.c_int, C.c_int, C.c_char, C.c_char, C.c_int64, C.c_int64,
                            C.c_int64, C.c_double, _vp, C.c_int64, _vp, C.c_int64, C.c_double,
                            _vp, C.c_int64]),

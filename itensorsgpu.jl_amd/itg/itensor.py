@@ -219,11 +219,171 @@ def cpu(x):
     return x.cpu()
 
 
+# ---------------------------------------------------------------- Diag
+class DiagITensor:
+    """Diag storage (NDTensors.Diag on the device, src/tensor/cudiag.jl):
+    only the diagonal of a tensor over `inds` is stored, as a device vector
+    of length min(dims); a plain number makes it a uniform diagonal
+    (NDTensors.Diag(2.0))."""
+
+    def __init__(self, inds, data):
+        self.inds = tuple(inds)
+        self.data = data
+
+    @property
+    def dims(self):
+        return tuple(i.dim for i in self.inds)
+
+    @property
+    def uniform(self):
+        return not hasattr(self.data, "numel")
+
+    ondevice = True
+
+    def is_complex(self):
+        if self.uniform:
+            return isinstance(self.data, complex)
+        return self.data.is_complex()
+
+    def complex(self):
+        if self.is_complex():
+            return self
+        if self.uniform:
+            return DiagITensor(self.inds, complex(self.data))
+        return DiagITensor(self.inds, ops.convert(self.data, "complex128"))
+
+    def _need_device(self):
+        pass
+
+    def dense(self, dtype=None):
+        """The dense tensor with this diagonal (NDTensors.dense)."""
+        dt = dtype or ("complex128" if self.is_complex() else "float64")
+        one = ops.fill(ops.empty(1, dt), 1.0)
+        lab = list(range(1, len(self.inds) + 1))
+        C, _ = ops.diag_contract(None if self.uniform else self.data, self.dims, lab, one, (), (),
+                                 lab, uniform=self.data if self.uniform else None, dtype=dt)
+        return ITensor(self.inds, C)
+
+    def array(self, *inds):
+        return self.dense().array(*inds)
+
+    def __mul__(self, other):
+        if isinstance(other, (int, float, complex, np.number)):
+            if self.uniform:
+                return DiagITensor(self.inds, self.data * other)
+            out = ops.copy(self.data)
+            if isinstance(other, (complex, np.complexfloating)) and not out.is_complex():
+                out = ops.convert(out, "complex128")
+            ops.scal(out, other)
+            return DiagITensor(self.inds, out)
+        return contract(self, other)
+
+    __rmul__ = lambda self, x: self.__mul__(x)
+
+
+def diag_itensor(values, *inds):
+    """Diag ITensor with the given diagonal (host array, uploaded) or a
+    uniform value (a number)."""
+    if np.isscalar(values):
+        return DiagITensor(inds, values)
+    v = np.asarray(values)
+    if v.size != min(i.dim for i in inds):
+        raise _lib.DimensionMismatch("diagonal length must be the smallest dimension")
+    return DiagITensor(inds, ops.to_device(v, "complex128" if np.iscomplexobj(v) else "float64"))
+
+
+def _as_complex(T):
+    return T.complex() if hasattr(T, "complex") else T
+
+
+def _contract_diag(A, B):
+    if A.is_complex() != B.is_complex():
+        A, B = _as_complex(A), _as_complex(B)
+    out_inds = [i for i in A.inds if i not in B.inds] + [i for i in B.inds if i not in A.inds]
+    for i in A.inds:
+        if i in B.inds and B.inds[B.inds.index(i)].dim != i.dim:
+            raise _lib.DimensionMismatch("contract: index dimension mismatch")
+    if isinstance(A, DiagITensor) and isinstance(B, DiagITensor):
+        # Diag x Diag: the diagonals multiply (cudiag.jl:118-140)
+        if A.uniform and B.uniform:
+            data = A.data * B.data
+        elif A.uniform or B.uniform:
+            u, v = (A.data, B.data) if A.uniform else (B.data, A.data)
+            data = ops.scal(ops.copy(v), u)
+        else:
+            if A.data.numel() != B.data.numel():
+                raise _lib.DimensionMismatch("Diag x Diag: diagonal lengths differ")
+            data = ops.hadamard(A.data, B.data)
+        if out_inds:
+            return DiagITensor(out_inds, data)
+        # every index contracted: the trace of the product
+        if not hasattr(data, "numel"):
+            return ITensor([], ops.to_device(np.array([data * min(A.dims)]),
+                                             "complex128" if isinstance(data, complex)
+                                             else "float64"))
+        ones = ops.fill(ops.empty(data.numel(), ops.dtype_name(data)), 1.0)
+        s = ops.dot(ones, data)
+        return ITensor([], ops.to_device(np.array([s]), ops.dtype_name(data)))
+    D, T = (A, B) if isinstance(A, DiagITensor) else (B, A)
+    T._need_device()
+    labels = {}
+    for i in list(A.inds) + list(B.inds):
+        labels.setdefault(i, len(labels) + 1)
+    C, _ = ops.diag_contract(None if D.uniform else D.data, D.dims, [labels[i] for i in D.inds],
+                             T.data, T.dims, [labels[i] for i in T.inds],
+                             [labels[i] for i in out_inds],
+                             uniform=D.data if D.uniform else None)
+    return ITensor(out_inds, C)
+
+
+# ------------------------------------------------------------ Combiner
+class Combiner:
+    """combiner(i, j, ...): fuses its indices into one index c (first index
+    fastest, NDTensors' combined-index order); contracting it with a tensor
+    holding all of them combines, with one holding c splits. On the device a
+    combine is one permutation (cucombiner.jl: Combiner x CuDense promotes
+    to CuDense)."""
+
+    def __init__(self, inds, tags="CMB,Link"):
+        self.uncombined = tuple(inds)
+        self.c = Index(int(np.prod([i.dim for i in inds])), tags)
+        self.inds = (self.c,) + self.uncombined
+
+    def combinedind(self):
+        return self.c
+
+
+def combiner(*inds, tags="CMB,Link"):
+    return Combiner(inds, tags)
+
+
+def _contract_combiner(A, B):
+    Cm, T = (A, B) if isinstance(A, Combiner) else (B, A)
+    T._need_device()
+    if all(i in T.inds for i in Cm.uncombined):
+        rest = [i for i in T.inds if i not in Cm.uncombined]
+        P = T.permute(*(list(Cm.uncombined) + rest))
+        return ITensor([Cm.c] + rest, P.data)
+    if Cm.c in T.inds:
+        rest = [i for i in T.inds if i != Cm.c]
+        P = T.permute(*([Cm.c] + rest))
+        return ITensor(list(Cm.uncombined) + rest, P.data)
+    raise _lib.DimensionMismatch("combiner: the tensor holds neither its indices nor the "
+                                 "combined index")
+
+
 # ------------------------------------------------------------- contraction
 def contract(A, B, mode="native"):
     """A*B: common indices summed; result indices = A's unique then B's
     unique (NDTensors contract_labels; contract!! dispatch of
-    src/tensor/cudense.jl:83-110, one GETT kernel for every branch)."""
+    src/tensor/cudense.jl:83-110, one GETT kernel for every branch). Diag
+    and Combiner operands take their own paths."""
+    if isinstance(A, DiagITensor) or isinstance(B, DiagITensor):
+        return _contract_diag(A, B)
+    if isinstance(A, Combiner) or isinstance(B, Combiner):
+        return _contract_combiner(A, B)
+    if A.is_complex() != B.is_complex():
+        A, B = A.complex(), B.complex()
     A._need_device()
     B._need_device()
     labels = {}

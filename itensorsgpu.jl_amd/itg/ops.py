@@ -194,6 +194,46 @@ def contract(A, dimsA, labelsA, B, dimsB, labelsB, labelsC, mode="native", alpha
     return out, dimsC
 
 
+def diag_contract(D, dimsD, labelsD, B, dimsB, labelsB, labelsC, uniform=None, alpha=1.0,
+                  beta=0.0, out=None, dtype=None):
+    """C = alpha * Diag(D) . B + beta * C; D is the device diagonal (length
+    min(dimsD)) or None with `uniform` a number (UniformDiag)."""
+    dmap = dict(zip(labelsB, dimsB))
+    dmap.update(zip(labelsD, dimsD))
+    dimsC = tuple(int(dmap[l]) for l in labelsC)
+    dt = dtype or dtype_name(B)
+    if out is None:
+        out = empty(prod(dimsC), dt)
+    u = None
+    if D is None:
+        z = complex(uniform)
+        u = (C.c_double * 2)(z.real, z.imag)
+    c = ctx()
+    c.check(lib().itg_diag_contract(c.ptr, dtype_code(B), len(dimsD), i64arr(dimsD),
+                                    i32arr(labelsD), ptr(D) if D is not None else None,
+                                    min(dimsD) if D is not None else 0, u,
+                                    len(dimsB), i64arr(dimsB), i32arr(labelsB), ptr(B),
+                                    len(labelsC), i64arr(dimsC), i32arr(labelsC), ptr(out),
+                                    float(alpha), float(beta)))
+    return out, dimsC
+
+
+def hadamard(x, y, out=None):
+    """out = x .* y element-wise."""
+    out = out if out is not None else empty(x.numel(), dtype_name(x))
+    c = ctx()
+    c.check(lib().itg_hadamard(c.ptr, dtype_code(x), x.numel(), ptr(x), ptr(y), ptr(out)))
+    return out
+
+
+def embed(S, dimsS, D, dimsD, offsets):
+    """Copy S into the block of D starting at `offsets` (in place on D)."""
+    c = ctx()
+    c.check(lib().itg_embed(c.ptr, dtype_code(S), len(dimsS), i64arr(dimsS), ptr(S),
+                            i64arr(dimsD), i64arr(offsets), ptr(D)))
+    return D
+
+
 def gemm(ta, tb, m, n, k, alpha, A, lda, B, ldb, beta, Cm, ldc, mode="native"):
     c = ctx()
     c.check(lib().itg_gemm(c.ptr, dtype_code(A), _lib.MODES[mode], ta.encode(), tb.encode(),

@@ -182,6 +182,27 @@ function randomRocMPS(sites::Vector{<:Index}; linkdims::Integer=1)
     return M
 end
 
+# ------------------------------------------------------------- Diag storage
+# Diag x Dense (src/tensor/cudiag.jl:118-162) through itg_diag_contract:
+# only the diagonal of D is read; a uniform Diag passes its value instead
+const RocDiag{ElT} = Diag{ElT,RocVector{ElT}}
+function contract!!(R::RocDenseTensor{ElT,NR}, labelsR, D::Tensor{ElT,ND,<:RocDiag{ElT}},
+                    labelsD, T::RocDenseTensor{ElT,NT}, labelsT) where {ElT,NR,ND,NT}
+    dD = Int64[dims(D)...]; dT = Int64[dims(T)...]; dC = Int64[dims(R)...]
+    lD = Int32[labelsD...]; lT = Int32[labelsT...]; lC = Int32[labelsR...]
+    check(ccall((:itg_diag_contract, libitg), Cint,
+                (Ptr{Cvoid}, Cint, Cint, Ptr{Int64}, Ptr{Int32}, Ptr{ElT}, Int64, Ptr{Cdouble},
+                 Cint, Ptr{Int64}, Ptr{Int32}, Ptr{ElT}, Cint, Ptr{Int64}, Ptr{Int32}, Ptr{ElT},
+                 Cdouble, Cdouble),
+                ctx(), dtype(ElT), ND, dD, lD, pointer(data(store(D))), Int64(minimum(dD)),
+                C_NULL, NT, dT, lT, pointer(data(store(T))), NR, dC, lC,
+                pointer(data(store(R))), 1.0, 0.0))
+    return R
+end
+contract!!(R::RocDenseTensor, labelsR, T::RocDenseTensor, labelsT,
+           D::Tensor{<:Number,<:Any,<:RocDiag}, labelsD) = contract!!(R, labelsR, D, labelsD,
+                                                                     T, labelsT)
+
 # ----------------------------------------------------------------- contract
 # Labels follow the einsum convention of itg_contract; NDTensors labels
 # (negative = contracted) map directly.
