@@ -312,8 +312,9 @@ def learned_form(N, chi, krylov, train_chi=64, steps_per_bond=10):
     the guarded network by the one-step Ritz value over three sweeps of a
     random chain at maxdim train_chi (seed 4), then time one sweep at maxdim
     chi from the same random MPS (seed 1) as the plain split-BF16 sweep. The
-    network runs in the host-driven Davidson (2 steps per bond, diagonal
-    fallback); its training is reported, not timed with the sweep."""
+    network runs inside the device Davidson (itg_davidson2_mlp: 2 steps per
+    bond, each learned step kept only if it lowers the residual, else the
+    diagonal step); its training is reported, not timed with the sweep."""
     import torch
     from itg import _lib
     from itg.mps import MPO, PhaseTimer, ProjMPO, Sweeps, dmrg, heisenberg_opsum, random_mps
@@ -335,7 +336,7 @@ def learned_form(N, chi, krylov, train_chi=64, steps_per_bond=10):
     _lib.ctx().sync()
     t1 = time.perf_counter()
     E = dmrg(H, psi, Sweeps(1, maxdim=[chi], cutoff=[1e-12]), mode="bf16x3", krylovdim=krylov,
-             ph=ph, timer=timer, record=rec, precond=net.factory(), native=False,
+             ph=ph, timer=timer, record=rec, precond=net.factory(),
              sweep_callback=lambda r: log("[learned] timed", r))
     torch.cuda.synchronize()
     _lib.ctx().sync()

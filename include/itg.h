@@ -304,6 +304,23 @@ int itg_davidson2(itg_ctx* ctx, int dtype, int mode, int64_t chiL, int64_t wl,
                   const void* R, const void* prep, void* psi, int maxiter,
                   double errgoal, const void* diag, void* work, void* kwork,
                   double* energy, int* matvecs);
+/* itg_davidson2 with the learned preconditioner (no counterpart in the
+ * reference, whose davidson has no preconditioner): each correction is
+ * first the network's (itg_mlp_features -> itg_mlp_infer -> itg_mlp_scatter
+ * on the residual, in = 3 d^2, out = d^2 per bond pair, plus the skip step
+ * at rms 1: skip 0 none, 1 the residual, 2 the diagonal step with `floor`)
+ * and is kept only if it lowers the residual norm; otherwise the basis
+ * vector is dropped, the previous Ritz pair restored and the diagonal step
+ * taken. `diag` (the Heff diagonal) is required. mwork:
+ * itg_davidson2_mlp_work_elems() floats; the rest as itg_davidson2.
+ * `reverted` counts the learned steps taken back. */
+int64_t itg_davidson2_mlp_work_elems(int64_t chiL, int64_t d, int64_t chiR, int hidden);
+int itg_davidson2_mlp(itg_ctx* ctx, int dtype, int mode, int64_t chiL, int64_t wl,
+                      int64_t wm, int64_t wr, int64_t d, int64_t chiR, const void* L,
+                      const void* R, const void* prep, void* psi, int maxiter,
+                      double errgoal, const void* diag, const float* params,
+                      int hidden, int skip, double floor, void* mwork, void* work,
+                      void* kwork, double* energy, int* matvecs, int* reverted);
 /* No counterpart in the reference: for callers that cannot allocate device
  * memory (code under HIP graph capture, tight serving loops, harnesses that
  * forbid hipMalloc). The same product as itg_heff2_apply, computed by the

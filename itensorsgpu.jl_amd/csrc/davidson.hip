@@ -209,11 +209,55 @@ extern "C" int64_t itg_davidson2_work_elems(int64_t chiL, int64_t d, int64_t chi
   return (int64_t)(2 * (maxiter + 1) + 3) * n + 64;
 }
 
-extern "C" int itg_davidson2(itg_ctx* ctx, int dtype, int mode, int64_t chiL, int64_t wl,
-                             int64_t wm, int64_t wr, int64_t d, int64_t chiR, const void* L,
-                             const void* R, const void* prep, void* psi, int maxiter,
-                             double errgoal, const void* diag, void* work, void* kwork,
-                             double* energy, int* matvecs) {
+// The learned Davidson preconditioner run inside the device loop: feature
+// rows of the residual, the fused network, scatter, plus the skip step
+// (0 none, 1 the residual, 2 the diagonal step), each skip at rms 1.
+struct LearnedStep {
+  const float* params = nullptr;
+  int in = 0, hidden = 0, out = 0;
+  int skip = 2;
+  double floor = 1e-8;
+  float* X = nullptr;       // rows x in
+  float* Y = nullptr;       // rows x out
+  float* work = nullptr;    // activations when the fused kernel does not fit
+  double* tskip = nullptr;  // n doubles
+};
+
+static int learned_correction(itg_ctx* ctx, const LearnedStep& ls, int64_t chiL, int64_t d,
+                              int64_t chiR, const double* q, const double* diag, double lam,
+                              double* t) {
+  const int64_t rows = chiL * chiR, n = rows * d * d;
+  int rc = itg_mlp_features(ctx, chiL, d, chiR, q, diag, lam, ls.floor, 0, 1, rows, ls.X);
+  if (rc) return rc;
+  rc = itg_mlp_infer(ctx, (int)rows, ls.in, ls.hidden, ls.out, ls.params, ls.X, ls.Y, ls.work);
+  if (rc) return rc;
+  rc = itg_mlp_scatter(ctx, chiL, d, chiR, ls.Y, 0, rows, t);
+  if (rc) return rc;
+  if (ls.skip == 0) return ITG_OK;
+  const double* sv = q;
+  if (ls.skip == 2) {
+    rc = itg_precond_diag(ctx, ITG_F64, n, q, diag, lam, ls.floor, ls.tskip);
+    if (rc) return rc;
+    sv = ls.tskip;
+  }
+  double h[kMaxBasis];
+  rc = multidot(ctx, n, sv, 1, sv, h);
+  if (rc) return rc;
+  const double sn = std::sqrt(h[0]);
+  double sc = sn > 0.0 ? std::sqrt((double)n) / sn : 0.0;
+  return lincomb(ctx, n, sv, 1, &sc, 1.0, t);
+}
+
+// ITensors' davidson on one bond with the basis on the device. With a
+// learned step, each correction is first taken with the network and kept
+// only if it lowers the residual norm; otherwise the basis vector is dropped,
+// the previous Ritz pair restored and the diagonal step taken (`reverted`
+// counts these).
+static int davidson2_core(itg_ctx* ctx, int dtype, int mode, int64_t chiL, int64_t wl, int64_t wm,
+                          int64_t wr, int64_t d, int64_t chiR, const void* L, const void* R,
+                          const void* prep, void* psi, int maxiter, double errgoal,
+                          const void* diag, const LearnedStep* ls, void* work, void* kwork,
+                          double* energy, int* matvecs, int* reverted) {
   ITG_CHECK_CTX(ctx);
   if (dtype != ITG_F64) return itg_fail(ctx, ITG_EDTYPE, "davidson2: FP64 vectors only");
   if (maxiter < 1 || maxiter + 1 > kMaxBasis)
@@ -225,6 +269,7 @@ extern "C" int itg_davidson2(itg_ctx* ctx, int dtype, int mode, int64_t chiL, in
   const int64_t n = chiL * d * d * chiR;
   if (n <= 0) return itg_fail(ctx, ITG_EDIM, "davidson2: empty vector");
   if (maxiter > n - 1) maxiter = (int)std::max<int64_t>(n - 1, 0);
+  const bool learned = ls != nullptr && ls->params != nullptr;
   double* V = (double*)kwork;                      // basis, maxiter + 1 vectors
   double* AV = V + (size_t)(maxiter + 1) * n;       // Heff applied to the basis
   double* q = AV + (size_t)(maxiter + 1) * n;       // residual
@@ -242,7 +287,7 @@ extern "C" int itg_davidson2(itg_ctx* ctx, int dtype, int mode, int64_t chiL, in
   if (rc) return rc;
   rc = heff_apply_graph(ctx, dtype, mode, dims, L, R, prep, V, AV, work);
   if (rc) return rc;
-  int nmv = 1, m = 1;
+  int nmv = 1, m = 1, nrev = 0;
   rc = multidot(ctx, n, V, 1, AV, h);
   if (rc) return rc;
   double lam = h[0], last = NAN;
@@ -257,54 +302,94 @@ extern "C" int itg_davidson2(itg_ctx* ctx, int dtype, int mode, int64_t chiL, in
     if (rc) return rc;
   }
   u[0] = 1.0;
+  rc = multidot(ctx, n, q, 1, q, h);
+  if (rc) return rc;
+  double qn = std::sqrt(h[0]);
   for (int ni = 1; ni <= maxiter; ++ni) {
-    rc = multidot(ctx, n, q, 1, q, h);
-    if (rc) return rc;
-    double qn = std::sqrt(h[0]);
     bool conv = (qn < errgoal && std::fabs(lam - last) < errgoal) ||
                 qn < std::max(1e-12, errgoal * 1e-3);
     if (qn < 1e-20 || (conv && ni > 1)) break;
     last = lam;
-    // correction
-    if (diag) rc = itg_precond_diag(ctx, ITG_F64, n, q, diag, lam, 1e-8, t);
-    else rc = itg_memcpy_d2d(ctx, t, q, (size_t)n * sizeof(double));
-    if (rc) return rc;
-    // two Gram-Schmidt passes against the basis
-    for (int pass = 0; pass < 2; ++pass) {
-      rc = multidot(ctx, n, V, m, t, h);
+    bool stepped = false;
+    const int attempts = learned ? 2 : 1;
+    for (int att = 0; att < attempts && !stepped; ++att) {
+      const bool net = learned && att == 0;
+      const bool guarded = net;   // the diagonal step is the fallback
+      // correction
+      if (net) rc = learned_correction(ctx, *ls, chiL, d, chiR, q, (const double*)diag, lam, t);
+      else if (diag) rc = itg_precond_diag(ctx, ITG_F64, n, q, diag, lam, 1e-8, t);
+      else rc = itg_memcpy_d2d(ctx, t, q, (size_t)n * sizeof(double));
       if (rc) return rc;
-      for (int k = 0; k < m; ++k) h[k] = -h[k];
-      rc = lincomb(ctx, n, V, m, h, 1.0, t);
+      // two Gram-Schmidt passes against the basis
+      for (int pass = 0; pass < 2; ++pass) {
+        rc = multidot(ctx, n, V, m, t, h);
+        if (rc) return rc;
+        for (int k = 0; k < m; ++k) h[k] = -h[k];
+        rc = lincomb(ctx, n, V, m, h, 1.0, t);
+        if (rc) return rc;
+      }
+      rc = multidot(ctx, n, t, 1, t, h);
       if (rc) return rc;
+      double tn = std::sqrt(h[0]);
+      if (tn < 1e-14 * std::max(1.0, qn)) {
+        if (guarded) {
+          ++nrev;
+          continue;
+        }
+        break;
+      }
+      // the Ritz pair before this step, restored if the step is reverted
+      const std::vector<double> Mprev(M.begin(), M.begin() + m * m);
+      const double lamprev = lam;
+      double uprev[kMaxBasis];
+      for (int k = 0; k < m; ++k) uprev[k] = u[k];
+      double sc = 1.0 / tn;
+      rc = lincomb(ctx, n, t, 1, &sc, 0.0, V + (size_t)m * n);
+      if (rc) return rc;
+      rc = heff_apply_graph(ctx, dtype, mode, dims, L, R, prep, V + (size_t)m * n,
+                            AV + (size_t)m * n, work);
+      if (rc) return rc;
+      ++nmv;
+      rc = multidot(ctx, n, V, m + 1, AV + (size_t)m * n, h);
+      if (rc) return rc;
+      ++m;
+      std::vector<double> Mn(m * m);
+      for (int a = 0; a < m - 1; ++a)
+        for (int b = 0; b < m - 1; ++b) Mn[a * m + b] = M[a * (m - 1) + b];
+      for (int k = 0; k < m; ++k) Mn[k * m + (m - 1)] = Mn[(m - 1) * m + k] = h[k];
+      M.assign(Mn.begin(), Mn.end());
+      small_lowest(m, M.data(), &lam, u);
+      // phi = sum u_k V_k, q = sum u_k AV_k - lam phi
+      rc = lincomb(ctx, n, V, m, u, 0.0, phi);
+      if (rc) return rc;
+      rc = lincomb(ctx, n, AV, m, u, 0.0, q);
+      if (rc) return rc;
+      double ml = -lam;
+      rc = lincomb(ctx, n, phi, 1, &ml, 1.0, q);
+      if (rc) return rc;
+      rc = multidot(ctx, n, q, 1, q, h);
+      if (rc) return rc;
+      const double qn_new = std::sqrt(h[0]);
+      if (guarded && !(qn_new < qn)) {
+        // the learned step did not lower the residual: drop it
+        --m;
+        M.assign(Mprev.begin(), Mprev.end());
+        lam = lamprev;
+        for (int k = 0; k < m; ++k) u[k] = uprev[k];
+        rc = lincomb(ctx, n, V, m, u, 0.0, phi);
+        if (rc) return rc;
+        rc = lincomb(ctx, n, AV, m, u, 0.0, q);
+        if (rc) return rc;
+        ml = -lam;
+        rc = lincomb(ctx, n, phi, 1, &ml, 1.0, q);
+        if (rc) return rc;
+        ++nrev;
+        continue;
+      }
+      qn = qn_new;
+      stepped = true;
     }
-    rc = multidot(ctx, n, t, 1, t, h);
-    if (rc) return rc;
-    double tn = std::sqrt(h[0]);
-    if (tn < 1e-14 * std::max(1.0, qn)) break;
-    double sc = 1.0 / tn;
-    rc = lincomb(ctx, n, t, 1, &sc, 0.0, V + (size_t)m * n);
-    if (rc) return rc;
-    rc = heff_apply_graph(ctx, dtype, mode, dims, L, R, prep, V + (size_t)m * n,
-                          AV + (size_t)m * n, work);
-    if (rc) return rc;
-    ++nmv;
-    rc = multidot(ctx, n, V, m + 1, AV + (size_t)m * n, h);
-    if (rc) return rc;
-    ++m;
-    std::vector<double> Mn(m * m);
-    for (int a = 0; a < m - 1; ++a)
-      for (int b = 0; b < m - 1; ++b) Mn[a * m + b] = M[a * (m - 1) + b];
-    for (int k = 0; k < m; ++k) Mn[k * m + (m - 1)] = Mn[(m - 1) * m + k] = h[k];
-    M.assign(Mn.begin(), Mn.end());
-    small_lowest(m, M.data(), &lam, u);
-    // phi = sum u_k V_k, q = sum u_k AV_k - lam phi
-    rc = lincomb(ctx, n, V, m, u, 0.0, phi);
-    if (rc) return rc;
-    rc = lincomb(ctx, n, AV, m, u, 0.0, q);
-    if (rc) return rc;
-    double ml = -lam;
-    rc = lincomb(ctx, n, phi, 1, &ml, 1.0, q);
-    if (rc) return rc;
+    if (!stepped) break;
   }
   // psi = Ritz vector (V0 when no step was taken), normalised
   const double* src = (m == 1) ? V : phi;
@@ -315,5 +400,60 @@ extern "C" int itg_davidson2(itg_ctx* ctx, int dtype, int mode, int64_t chiL, in
   if (rc) return rc;
   *energy = lam;
   if (matvecs) *matvecs = nmv;
+  if (reverted) *reverted = nrev;
   return ITG_OK;
+}
+
+extern "C" int itg_davidson2(itg_ctx* ctx, int dtype, int mode, int64_t chiL, int64_t wl,
+                             int64_t wm, int64_t wr, int64_t d, int64_t chiR, const void* L,
+                             const void* R, const void* prep, void* psi, int maxiter,
+                             double errgoal, const void* diag, void* work, void* kwork,
+                             double* energy, int* matvecs) {
+  return davidson2_core(ctx, dtype, mode, chiL, wl, wm, wr, d, chiR, L, R, prep, psi, maxiter,
+                        errgoal, diag, nullptr, work, kwork, energy, matvecs, nullptr);
+}
+
+static bool mlp_fused(int in, int hidden, int out) {
+  return hidden == 256 && ((in <= 16 && out <= 4) || (in <= 32 && out <= 9));
+}
+
+extern "C" int64_t itg_davidson2_mlp_work_elems(int64_t chiL, int64_t d, int64_t chiR,
+                                                int hidden) {
+  const int64_t rows = chiL * chiR, in = 3 * d * d, out = d * d;
+  int64_t f = rows * in + rows * out + 2 * rows * d * d + 64;
+  if (!mlp_fused((int)in, hidden, (int)out)) f += 2 * rows * (int64_t)hidden;
+  return f;
+}
+
+extern "C" int itg_davidson2_mlp(itg_ctx* ctx, int dtype, int mode, int64_t chiL, int64_t wl,
+                                 int64_t wm, int64_t wr, int64_t d, int64_t chiR, const void* L,
+                                 const void* R, const void* prep, void* psi, int maxiter,
+                                 double errgoal, const void* diag, const float* params,
+                                 int hidden, int skip, double floor, void* mwork, void* work,
+                                 void* kwork, double* energy, int* matvecs, int* reverted) {
+  ITG_CHECK_CTX(ctx);
+  if (!diag || !params || !mwork)
+    return itg_fail(ctx, ITG_EARG, "davidson2_mlp: diagonal, parameters and work are required");
+  if (skip < 0 || skip > 2) return itg_fail(ctx, ITG_EARG, "davidson2_mlp: skip %d", skip);
+  if (!(floor > 0.0)) return itg_fail(ctx, ITG_EARG, "davidson2_mlp: floor must be > 0");
+  const int64_t rows = chiL * chiR;
+  if (rows <= 0 || rows > 0x7fffffff) return itg_fail(ctx, ITG_EDIM, "davidson2_mlp: rows");
+  LearnedStep ls;
+  ls.params = params;
+  ls.in = (int)(3 * d * d);
+  ls.hidden = hidden;
+  ls.out = (int)(d * d);
+  ls.skip = skip;
+  ls.floor = floor;
+  // work: X | Y | tskip (n doubles, 8-byte aligned) | activations
+  float* w = (float*)mwork;
+  ls.X = w;
+  ls.Y = ls.X + rows * ls.in;
+  size_t off = (size_t)(rows * ls.in + rows * ls.out);
+  off = (off + 1) & ~(size_t)1;
+  ls.tskip = (double*)(w + off);
+  off += 2 * (size_t)rows * d * d;
+  ls.work = mlp_fused(ls.in, hidden, ls.out) ? nullptr : w + off;
+  return davidson2_core(ctx, dtype, mode, chiL, wl, wm, wr, d, chiR, L, R, prep, psi, maxiter,
+                        errgoal, diag, &ls, work, kwork, energy, matvecs, reverted);
 }

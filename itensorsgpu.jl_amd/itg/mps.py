@@ -418,7 +418,7 @@ class ProjMPO:
                                 prep=buf)
         self._prep = (b, self.mode, dims, buf, L, R)
 
-    def eigsolve(self, b, phi, xdims, maxiter=2, errgoal=1e-14, diag=None, stats=None):
+    def eigsolve(self, b, phi, xdims, maxiter=2, errgoal=1e-14, diag=None, stats=None, net=None):
         """Native device Davidson (itg_davidson2) on the prepared bond b;
         phi is overwritten with the Ritz vector. The work buffers are the
         ProjMPO's grow-only ones, so bonds of one shape replay the split-BF16
@@ -435,13 +435,25 @@ class ProjMPO:
                                                                            dims[5], maxiter),
                             "float64")
         ev = self.timer.start() if self.timer is not None else None
-        lam, nmv = ops.davidson2(L, R, p[3], phi, *dims, maxiter=maxiter, errgoal=errgoal,
-                                 diag=diag, mode=self.mode, work=work, kwork=kwork)
+        nrev = 0
+        if net is not None:
+            # the guarded learned correction inside the device loop
+            mwork = self.buffer("mwork", _lib.lib().itg_davidson2_mlp_work_elems(
+                dims[0], dims[4], dims[5], net.h), "float32")
+            lam, nmv, nrev = ops.davidson2_mlp(L, R, p[3], phi, *dims, diag=diag,
+                                               params=net.params, hidden=net.h, skip=net.skip,
+                                               floor=net.floor, maxiter=maxiter, errgoal=errgoal,
+                                               mode=self.mode, work=work, kwork=kwork,
+                                               mwork=mwork)
+        else:
+            lam, nmv = ops.davidson2(L, R, p[3], phi, *dims, maxiter=maxiter, errgoal=errgoal,
+                                     diag=diag, mode=self.mode, work=work, kwork=kwork)
         if ev is not None:
             self.timer.stop("davidson", ev, nmv * heff_flops(dims[0], dims[1], dims[3], dims[4],
                                                              dims[5]))
         if stats is not None:
             stats["matvecs"] = stats.get("matvecs", 0) + nmv
+            stats["reverted"] = stats.get("reverted", 0) + nrev
         return lam, phi
 
     def product(self, b, x, xdims, out=None, work=None):
@@ -799,12 +811,15 @@ def dmrg(H, psi, sweeps, mode="native", krylovdim=3, precond=None, errgoal=1e-14
             B, Bd = psi.data[b + 1], psi.dims[b + 1]
             phi, pdims = ops.contract(A, Ad, [1, 2, 3], B, Bd, [3, 4, 5], [1, 2, 4, 5])
             PH.prepare(b, pdims)
-            if native and not callable(precond):
-                # device Davidson (itg_davidson2); phi is a fresh contraction
+            # a learned preconditioner that the device loop can run itself
+            net = getattr(precond, "device_net", None) if callable(precond) else None
+            if native and (not callable(precond) or net is not None):
+                # device Davidson (itg_davidson2, or itg_davidson2_mlp with
+                # the guarded learned step); phi is a fresh contraction
                 # output, overwritten with the Ritz vector
-                dg = PH.diag(b, pdims) if precond == "diag" else None
+                dg = PH.diag(b, pdims) if (precond == "diag" or net is not None) else None
                 energy, phi = PH.eigsolve(b, phi, pdims, maxiter=krylovdim - 1,
-                                          errgoal=errgoal, diag=dg, stats=stats)
+                                          errgoal=errgoal, diag=dg, stats=stats, net=net)
             else:
                 work = PH.heff_work((pdims[0], H.dims[b][0], H.dims[b][1], H.dims[b + 1][1],
                                      pdims[1], pdims[3]))

@@ -404,6 +404,35 @@ def davidson2(L, R, prep, psi, chiL, wl, wm, wr, d, chiR, maxiter=2, errgoal=1e-
     return e.value, nmv.value
 
 
+SKIPS = {"none": 0, "plain": 1, "diag": 2}
+
+
+def davidson2_mlp_work(chiL, d, chiR, hidden):
+    return empty(lib().itg_davidson2_mlp_work_elems(chiL, d, chiR, int(hidden)), "float32")
+
+
+def davidson2_mlp(L, R, prep, psi, chiL, wl, wm, wr, d, chiR, diag, params, hidden, skip="diag",
+                  floor=1e-8, maxiter=2, errgoal=1e-14, mode="native", work=None, kwork=None,
+                  mwork=None):
+    """Device Davidson with the guarded learned correction (itg_davidson2_mlp).
+    Returns (energy, matvecs, reverted)."""
+    if work is None:
+        work = heff2_work(chiL, wl, wm, wr, d, chiR, "float64")
+    if kwork is None:
+        kwork = davidson2_work(chiL, d, chiR, maxiter)
+    if mwork is None:
+        mwork = davidson2_mlp_work(chiL, d, chiR, hidden)
+    e = C.c_double()
+    nmv, nrev = C.c_int(), C.c_int()
+    c = ctx()
+    c.check(lib().itg_davidson2_mlp(c.ptr, dtype_code(psi), _lib.MODES[mode], chiL, wl, wm, wr,
+                                    d, chiR, ptr(L), ptr(R), ptr(prep), ptr(psi), int(maxiter),
+                                    float(errgoal), ptr(diag), ptr(params), int(hidden),
+                                    SKIPS[skip], float(floor), ptr(mwork), ptr(work), ptr(kwork),
+                                    C.byref(e), C.byref(nmv), C.byref(nrev)))
+    return e.value, nmv.value, nrev.value
+
+
 def heff2_diag(L, W1, W2, R, chiL, wl, wm, wr, d, chiR):
     dt = str(L.dtype).split(".")[-1]
     out = empty(chiL * d * d * chiR, dt)

@@ -239,6 +239,10 @@ class MLPPreconditioner:
                 pre.fallback = dpre
             return pre
 
+        # the device Davidson runs the same guarded step itself
+        # (itg_davidson2_mlp) when nothing else is asked of the factory
+        if guard and min_residual <= 0.0:
+            make.device_net = self
         return make
 
 

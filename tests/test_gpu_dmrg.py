@@ -282,6 +282,76 @@ def test_davidson2_matches_host_davidson(itg, mode, precond):
         assert abs(ov - 1.0) < 1e-9, (maxiter, ov)
 
 
+@pytest.mark.parametrize("mode", ["native", "bf16x3"])
+def test_davidson2_mlp_matches_host_guarded_davidson(itg, mode):
+    """itg_davidson2_mlp (the guarded learned step inside the device loop)
+    against the host Davidson with the same network correction and the
+    diagonal fallback, for a network whose layers run as FP32 products
+    (hidden 32): same Ritz value, vector, matvec and revert counts."""
+    from itg import ops
+    from itg.mps import davidson, diag_preconditioner
+    from itg.precond import MLPPreconditioner
+    PH, b, phi, pdims = _bond_setup(mode=mode)
+    PH.prepare(b, pdims)
+    dg = PH.diag(b, pdims)
+    net = MLPPreconditioner(2, hidden=32, seed=3)
+    net.set_params(np.random.default_rng(3).standard_normal(net.nparam).astype(np.float32) * 0.3)
+    pre = lambda q, lam: net.correction(q, lam, dg, pdims)
+    for maxiter in (1, 3, 6):
+        st_h, st_n = {}, {}
+        lam_h, v_h = davidson(lambda x: PH.product(b, x, pdims), ops.copy(phi), maxiter=maxiter,
+                              precond=pre, fallback=diag_preconditioner(dg), stats=st_h)
+        v_n = ops.copy(phi)
+        lam_n, v_n = PH.eigsolve(b, v_n, pdims, maxiter=maxiter, diag=dg, stats=st_n, net=net)
+        assert abs(lam_n - lam_h) < 1e-10 * abs(lam_h), (maxiter, lam_n, lam_h)
+        assert abs(abs(ops.dot(v_n, v_h)) - 1.0) < 1e-9
+        assert st_n["matvecs"] == st_h["matvecs"] and st_n["reverted"] == st_h["reverted"], \
+            (maxiter, st_n, st_h)
+
+
+def test_davidson2_mlp_zero_network_is_the_diagonal_run(itg):
+    """A zero network without skip gives a zero correction: every learned
+    step is taken back and the device run is the diagonal-preconditioned
+    itg_davidson2 run bit for bit."""
+    from itg import ops
+    from itg.precond import MLPPreconditioner
+    PH, b, phi, pdims = _bond_setup()
+    PH.prepare(b, pdims)
+    dg = PH.diag(b, pdims)
+    net = MLPPreconditioner(2, hidden=256, skip="none")
+    net.set_params(np.zeros(net.nparam, dtype=np.float32))
+    st = {}
+    v_n = ops.copy(phi)
+    lam_n, v_n = PH.eigsolve(b, v_n, pdims, maxiter=3, diag=dg, stats=st, net=net)
+    v_d = ops.copy(phi)
+    lam_d, v_d = PH.eigsolve(b, v_d, pdims, maxiter=3, diag=dg)
+    assert st["reverted"] == 3
+    assert lam_n == lam_d
+    assert np.array_equal(ops.to_host(v_n), ops.to_host(v_d))
+
+
+def test_dmrg_learned_device_path_matches_host(itg):
+    """dmrg with the trained network's factory on the device Davidson
+    (itg_davidson2_mlp) and on the host Davidson: both converge to the same
+    energy (N = 12 Heisenberg, chi 32, 4 sweeps), and the device run counts
+    its reverted steps."""
+    from itg.mps import MPO, Sweeps, dmrg, heisenberg_opsum, random_mps
+    from itg.precond import MLPPreconditioner, train_sweep
+    N = 12
+    H = MPO.from_opsum(heisenberg_opsum([(j, j + 1) for j in range(N - 1)]), N)
+    net = MLPPreconditioner(2, hidden=256, seed=1)
+    train_sweep(net, H, random_mps(N, 2, 16, seed=4), 16, steps_per_bond=3, sweeps=1, lr=3e-4)
+    sw = Sweeps(4, maxdim=[32], cutoff=[1e-12])
+    rec_d, rec_h = [], []
+    Ed = dmrg(H, random_mps(N, 2, 16, seed=2), sw, precond=net.factory(), record=rec_d)
+    Eh = dmrg(H, random_mps(N, 2, 16, seed=2), sw, precond=net.factory(), native=False,
+              record=rec_h)
+    assert abs(Ed - Eh) < 1e-9 * abs(Eh)
+    E0 = models.ground_energy(models.bond_hamiltonian(N, [(j, j + 1) for j in range(N - 1)]))
+    assert abs(Ed - E0) < 1e-8
+    assert all("reverted" in r for r in rec_d)
+
+
 def test_davidson2_graph_replays_match_eager(itg):
     """Split-BF16 Heff applies are captured into HIP graphs on their second
     use: five calls with eager library work and synchronisations between
