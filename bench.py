@@ -313,8 +313,9 @@ def learned_form(N, chi, krylov, train_chi=64, steps_per_bond=10):
     random chain at maxdim train_chi (seed 4), then time one sweep at maxdim
     chi from the same random MPS (seed 1) as the plain split-BF16 sweep. The
     network runs inside the device Davidson (itg_davidson2_mlp: 2 steps per
-    bond, each learned step kept only if it lowers the residual, else the
-    diagonal step); its training is reported, not timed with the sweep."""
+    bond, the first one learned and kept only if it lowers the residual, else
+    the diagonal step; the second the diagonal step); its training is
+    reported, not timed with the sweep."""
     import torch
     from itg import _lib
     from itg.mps import MPO, PhaseTimer, ProjMPO, Sweeps, dmrg, heisenberg_opsum, random_mps
@@ -324,6 +325,7 @@ def learned_form(N, chi, krylov, train_chi=64, steps_per_bond=10):
     net = MLPPreconditioner(2, hidden=256, seed=1)
     train_sweep(net, H, random_mps(N, 2, train_chi, seed=4), train_chi,
                 steps_per_bond=steps_per_bond, sweeps=3, lr=3e-4)
+    net.net_steps = 1      # the network on the first Davidson step of a bond
     _lib.ctx().sync()
     train_s = time.perf_counter() - t0
     log("[learned] trained: %d Adam steps (%.1f s)" % (net.step, train_s))

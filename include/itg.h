@@ -311,7 +311,8 @@ int itg_davidson2(itg_ctx* ctx, int dtype, int mode, int64_t chiL, int64_t wl,
  * at rms 1: skip 0 none, 1 the residual, 2 the diagonal step with `floor`)
  * and is kept only if it lowers the residual norm; otherwise the basis
  * vector is dropped, the previous Ritz pair restored and the diagonal step
- * taken. `diag` (the Heff diagonal) is required. mwork:
+ * taken. Only the first `net_steps` steps of a call try the network (0:
+ * every step). `diag` (the Heff diagonal) is required. mwork:
  * itg_davidson2_mlp_work_elems() floats; the rest as itg_davidson2.
  * `reverted` counts the learned steps taken back. */
 int64_t itg_davidson2_mlp_work_elems(int64_t chiL, int64_t d, int64_t chiR, int hidden);
@@ -319,8 +320,9 @@ int itg_davidson2_mlp(itg_ctx* ctx, int dtype, int mode, int64_t chiL, int64_t w
                       int64_t wm, int64_t wr, int64_t d, int64_t chiR, const void* L,
                       const void* R, const void* prep, void* psi, int maxiter,
                       double errgoal, const void* diag, const float* params,
-                      int hidden, int skip, double floor, void* mwork, void* work,
-                      void* kwork, double* energy, int* matvecs, int* reverted);
+                      int hidden, int skip, int net_steps, double floor, void* mwork,
+                      void* work, void* kwork, double* energy, int* matvecs,
+                      int* reverted);
 /* No counterpart in the reference: for callers that cannot allocate device
  * memory (code under HIP graph capture, tight serving loops, harnesses that
  * forbid hipMalloc). The same product as itg_heff2_apply, computed by the

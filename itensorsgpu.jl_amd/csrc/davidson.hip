@@ -216,6 +216,7 @@ struct LearnedStep {
   const float* params = nullptr;
   int in = 0, hidden = 0, out = 0;
   int skip = 2;
+  int net_steps = 0;        // Davidson steps per call that try the network (0: all)
   double floor = 1e-8;
   float* X = nullptr;       // rows x in
   float* Y = nullptr;       // rows x out
@@ -311,9 +312,10 @@ static int davidson2_core(itg_ctx* ctx, int dtype, int mode, int64_t chiL, int64
     if (qn < 1e-20 || (conv && ni > 1)) break;
     last = lam;
     bool stepped = false;
-    const int attempts = learned ? 2 : 1;
+    const bool try_net = learned && (ls->net_steps <= 0 || ni <= ls->net_steps);
+    const int attempts = try_net ? 2 : 1;
     for (int att = 0; att < attempts && !stepped; ++att) {
-      const bool net = learned && att == 0;
+      const bool net = try_net && att == 0;
       const bool guarded = net;   // the diagonal step is the fallback
       // correction
       if (net) rc = learned_correction(ctx, *ls, chiL, d, chiR, q, (const double*)diag, lam, t);
@@ -429,8 +431,9 @@ extern "C" int itg_davidson2_mlp(itg_ctx* ctx, int dtype, int mode, int64_t chiL
                                  int64_t wm, int64_t wr, int64_t d, int64_t chiR, const void* L,
                                  const void* R, const void* prep, void* psi, int maxiter,
                                  double errgoal, const void* diag, const float* params,
-                                 int hidden, int skip, double floor, void* mwork, void* work,
-                                 void* kwork, double* energy, int* matvecs, int* reverted) {
+                                 int hidden, int skip, int net_steps, double floor, void* mwork,
+                                 void* work, void* kwork, double* energy, int* matvecs,
+                                 int* reverted) {
   ITG_CHECK_CTX(ctx);
   if (!diag || !params || !mwork)
     return itg_fail(ctx, ITG_EARG, "davidson2_mlp: diagonal, parameters and work are required");
@@ -444,6 +447,7 @@ extern "C" int itg_davidson2_mlp(itg_ctx* ctx, int dtype, int mode, int64_t chiL
   ls.hidden = hidden;
   ls.out = (int)(d * d);
   ls.skip = skip;
+  ls.net_steps = net_steps;
   ls.floor = floor;
   // work: X | Y | tskip (n doubles, 8-byte aligned) | activations
   float* w = (float*)mwork;

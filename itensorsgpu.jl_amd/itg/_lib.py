@@ -119,7 +119,7 @@ _SIGS = {
     "itg_davidson2_mlp_work_elems": (C.c_int64, [C.c_int64, C.c_int64, C.c_int64, C.c_int]),
     "itg_davidson2_mlp": (C.c_int, [_ctxp, C.c_int, C.c_int] + [C.c_int64] * 6 +
                           [_vp, _vp, _vp, _vp, C.c_int, C.c_double, _vp, _vp, C.c_int, C.c_int,
-                           C.c_double, _vp, _vp, _vp, _dp, C.POINTER(C.c_int),
+                           C.c_int, C.c_double, _vp, _vp, _vp, _dp, C.POINTER(C.c_int),
                            C.POINTER(C.c_int)]),
     "itg_mlp_param_count": (C.c_int64, [C.c_int, C.c_int, C.c_int]),
     "itg_mlp_forward": (C.c_int, [_ctxp, C.c_int, C.c_int, C.c_int, C.c_int, _vp, _vp, _vp,

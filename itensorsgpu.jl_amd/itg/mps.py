@@ -444,7 +444,8 @@ class ProjMPO:
                                                params=net.params, hidden=net.h, skip=net.skip,
                                                floor=net.floor, maxiter=maxiter, errgoal=errgoal,
                                                mode=self.mode, work=work, kwork=kwork,
-                                               mwork=mwork)
+                                               mwork=mwork,
+                                               net_steps=getattr(net, "net_steps", 0))
         else:
             lam, nmv = ops.davidson2(L, R, p[3], phi, *dims, maxiter=maxiter, errgoal=errgoal,
                                      diag=diag, mode=self.mode, work=work, kwork=kwork)

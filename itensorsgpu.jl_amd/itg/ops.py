@@ -413,9 +413,10 @@ def davidson2_mlp_work(chiL, d, chiR, hidden):
 
 def davidson2_mlp(L, R, prep, psi, chiL, wl, wm, wr, d, chiR, diag, params, hidden, skip="diag",
                   floor=1e-8, maxiter=2, errgoal=1e-14, mode="native", work=None, kwork=None,
-                  mwork=None):
-    """Device Davidson with the guarded learned correction (itg_davidson2_mlp).
-    Returns (energy, matvecs, reverted)."""
+                  mwork=None, net_steps=0):
+    """Device Davidson with the guarded learned correction (itg_davidson2_mlp);
+    the first `net_steps` steps try the network (0: all). Returns (energy,
+    matvecs, reverted)."""
     if work is None:
         work = heff2_work(chiL, wl, wm, wr, d, chiR, "float64")
     if kwork is None:
@@ -428,7 +429,8 @@ def davidson2_mlp(L, R, prep, psi, chiL, wl, wm, wr, d, chiR, diag, params, hidd
     c.check(lib().itg_davidson2_mlp(c.ptr, dtype_code(psi), _lib.MODES[mode], chiL, wl, wm, wr,
                                     d, chiR, ptr(L), ptr(R), ptr(prep), ptr(psi), int(maxiter),
                                     float(errgoal), ptr(diag), ptr(params), int(hidden),
-                                    SKIPS[skip], float(floor), ptr(mwork), ptr(work), ptr(kwork),
+                                    SKIPS[skip], int(net_steps), float(floor), ptr(mwork),
+                                    ptr(work), ptr(kwork),
                                     C.byref(e), C.byref(nmv), C.byref(nrev)))
     return e.value, nmv.value, nrev.value
 

@@ -220,6 +220,10 @@ class MLPPreconditioner:
             ops.mlp_adam(self.params, self.grad, self.m, self.v, self.step, lr=lr)
         return mu - lam
 
+    # Davidson steps per bond that try the network on the device path (0:
+    # every step); the first step carries most of the gain
+    net_steps = 0
+
     def factory(self, guard=True, min_residual=0.0):
         """dmrg `precond` factory: (ProjMPO, b, xdims) -> precond(q, lam)
         carrying the diagonal preconditioner as its revert fallback. Residuals
