@@ -55,8 +55,9 @@ enum itg_status {
  * 0), itg_convert (to and from ITG_F64: real part / zero imaginary part),
  * itg_randn, itg_permute, itg_add_permuted, itg_scal, itg_axpby (real
  * coefficients), itg_dot (conj(x).y into out[0], out[1]), itg_nrm2,
- * itg_contract (ITG_MODE_NATIVE, ranks up to ITG_MAX_RANK - 2) and
- * itg_eigh_trunc (Hermitian); counts n are complex elements. */
+ * itg_contract (ITG_MODE_NATIVE, ranks up to ITG_MAX_RANK - 2),
+ * itg_eigh_trunc (Hermitian), itg_svd_trunc (A = U diag(S) V^H) and itg_qr
+ * (Q unitary); counts n are complex elements. */
 enum itg_dtype { ITG_F64 = 0, ITG_F32 = 1, ITG_C64 = 2 };
 enum itg_mode { ITG_MODE_NATIVE = 0, ITG_MODE_F32 = 1, ITG_MODE_BF16X3 = 2 };
 

@@ -489,3 +489,374 @@ int itg_zunmtr(itg_ctx* ctx, int n, int k, const void* Av, const void* tauv, con
   }
   return ITG_OK;
 }
+
+// ==================================================================== QR
+// Householder QR of a complex m x n matrix (zgeqr2): reflector j from
+// column j, rows j..m-1 (v(j) = 1 implicit, beta real on the diagonal), and
+// H_j^H applied to the later columns; Q = H_0 .. H_{r-1} [I; 0] formed by
+// one workgroup per column of Q with the column held in LDS.
+namespace {
+
+__global__ __launch_bounds__(256) void zqr_reflect_kernel(double2* __restrict__ W, int m, int j,
+                                                          double2* tau) {
+  __shared__ double2 sh[4];
+  double2* col = W + (size_t)j * m;
+  double s = 0.0;
+  for (int r = j + 1 + threadIdx.x; r < m; r += 256) {
+    double2 v = col[r];
+    s += v.x * v.x + v.y * v.y;
+  }
+  const double xn2 = block_csum_256(make_double2(s, 0.0), sh).x;
+  const double2 alpha = col[j];
+  __syncthreads();   // alpha read by every thread before it is overwritten
+  double beta = alpha.x;
+  double2 t = make_double2(0.0, 0.0), sc = make_double2(1.0, 0.0);
+  if (!(xn2 == 0.0 && alpha.y == 0.0)) {
+    beta = -copysign(sqrt(alpha.x * alpha.x + alpha.y * alpha.y + xn2), alpha.x);
+    t = make_double2((beta - alpha.x) / beta, -alpha.y / beta);
+    double2 dd = make_double2(alpha.x - beta, alpha.y);
+    double den = dd.x * dd.x + dd.y * dd.y;
+    sc = make_double2(dd.x / den, -dd.y / den);
+  }
+  for (int r = j + 1 + threadIdx.x; r < m; r += 256) col[r] = cmul(col[r], sc);
+  if (threadIdx.x == 0) {
+    col[j] = make_double2(beta, 0.0);
+    tau[j] = t;
+  }
+}
+
+// W(j:m, c) -= conj(tau_j) v (v^H W(j:m, c)) for c > j, one block per column
+__global__ __launch_bounds__(256) void zqr_apply_kernel(double2* __restrict__ W, int m, int n,
+                                                        int j, const double2* tau) {
+  __shared__ double2 sh[4];
+  const int c = j + 1 + blockIdx.x;
+  if (c >= n) return;
+  const double2 tj = tau[j];
+  if (tj.x == 0.0 && tj.y == 0.0) return;
+  const double2* v = W + (size_t)j * m;
+  double2* col = W + (size_t)c * m;
+  double2 s = make_double2(0.0, 0.0);
+  for (int r = j + threadIdx.x; r < m; r += 256) {
+    double2 p = r == j ? col[r] : cmulc(v[r], col[r]);
+    s.x += p.x;
+    s.y += p.y;
+  }
+  s = block_csum_256(s, sh);
+  const double2 f = cmulc(tj, s);   // conj(tau) (v^H col)
+  for (int r = j + threadIdx.x; r < m; r += 256) {
+    double2 p = r == j ? f : cmul(v[r], f);
+    col[r].x -= p.x;
+    col[r].y -= p.y;
+  }
+}
+
+// column c of Q = H_0 .. H_{r-1} e_c, the column in LDS
+__global__ __launch_bounds__(256) void zqr_form_q_kernel(const double2* __restrict__ W, int m,
+                                                         int r, const double2* tau,
+                                                         double2* __restrict__ Q) {
+  extern __shared__ double2 u[];
+  __shared__ double2 sh[4];
+  const int c = blockIdx.x;
+  for (int i = threadIdx.x; i < m; i += 256) u[i] = make_double2(i == c ? 1.0 : 0.0, 0.0);
+  __syncthreads();
+  for (int j = std::min(c, r - 1); j >= 0; --j) {
+    // H_j e_c vanishes above row j: reflectors past c leave the column as is
+    const double2* v = W + (size_t)j * m;
+    double2 s = make_double2(0.0, 0.0);
+    for (int i = j + threadIdx.x; i < m; i += 256) {
+      double2 p = i == j ? u[i] : cmulc(v[i], u[i]);
+      s.x += p.x;
+      s.y += p.y;
+    }
+    const double2 f = cmul(tau[j], block_csum_256(s, sh));
+    for (int i = j + threadIdx.x; i < m; i += 256) {
+      double2 p = i == j ? f : cmul(v[i], f);
+      u[i].x -= p.x;
+      u[i].y -= p.y;
+    }
+    __syncthreads();
+  }
+  for (int i = threadIdx.x; i < m; i += 256) Q[(size_t)c * m + i] = u[i];
+}
+
+__global__ void zqr_extract_r_kernel(int m, int n, int r, const double2* __restrict__ W,
+                                     double2* __restrict__ R) {
+  int64_t tot = (int64_t)r * n;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < tot;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    int i = (int)(e % r), c = (int)(e / r);
+    R[e] = i <= c ? W[i + (size_t)c * m] : make_double2(0.0, 0.0);
+  }
+}
+
+// ------------------------------------------------------- complex Jacobi
+// One round of the one-sided Jacobi SVD on the columns of W (m rows): pair
+// (p, q) of the round-robin schedule is rotated so that W(:,p)^H W(:,q) = 0,
+// the same rotation applied to V's columns: with g = a^H b = |g| e^{i phi},
+// a' = c a - s e^{-i phi} b, b' = s e^{i phi} a + c b (the real rotation of
+// (a, e^{-i phi} b)).
+__global__ __launch_bounds__(256) void zjac_round_kernel(double2* __restrict__ W, int m,
+                                                         double2* __restrict__ V, int nv,
+                                                         int ncols, int N, int rnd, double tol,
+                                                         double small2, double* __restrict__ offs) {
+  __shared__ double2 sh[4];
+  const int i = blockIdx.x;
+  int p, q;
+  if (i == 0) { p = rnd; q = N - 1; }
+  else { p = (rnd + i) % (N - 1); q = (rnd - i + (N - 1)) % (N - 1); }
+  if (p >= ncols || q >= ncols) return;
+  double2* a = W + (size_t)p * m;
+  double2* b = W + (size_t)q * m;
+  double sa = 0.0, sb = 0.0;
+  double2 g = make_double2(0.0, 0.0);
+  for (int r = threadIdx.x; r < m; r += 256) {
+    double2 x = a[r], y = b[r];
+    sa += x.x * x.x + x.y * x.y;
+    sb += y.x * y.x + y.y * y.y;
+    double2 pr = cmulc(x, y);
+    g.x += pr.x;
+    g.y += pr.y;
+  }
+  const double2 ab = block_csum_256(make_double2(sa, sb), sh);
+  sa = ab.x;
+  sb = ab.y;
+  g = block_csum_256(g, sh);
+  const double ga = sqrt(g.x * g.x + g.y * g.y);
+  const double rel = (sa > small2 && sb > small2) ? ga / sqrt(sa * sb) : 0.0;
+  if (threadIdx.x == 0) offs[i] = fmax(offs[i], rel);
+  if (rel <= tol) return;
+  const double2 ph = make_double2(g.x / ga, g.y / ga);    // e^{i phi}
+  const double zeta = (sb - sa) / (2.0 * ga);
+  const double t = copysign(1.0, zeta) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
+  const double c = 1.0 / sqrt(1.0 + t * t), s = c * t;
+  const double2 sp = make_double2(s * ph.x, s * ph.y);      // s e^{i phi}
+  for (int r = threadIdx.x; r < m; r += 256) {
+    double2 x = a[r], y = b[r];
+    double2 u1 = cmulc(sp, y);   // s e^{-i phi} y
+    double2 u2 = cmul(sp, x);    // s e^{i phi} x
+    a[r] = make_double2(c * x.x - u1.x, c * x.y - u1.y);
+    b[r] = make_double2(u2.x + c * y.x, u2.y + c * y.y);
+  }
+  double2* va = V + (size_t)p * nv;
+  double2* vb = V + (size_t)q * nv;
+  for (int r = threadIdx.x; r < nv; r += 256) {
+    double2 x = va[r], y = vb[r];
+    double2 u1 = cmulc(sp, y);
+    double2 u2 = cmul(sp, x);
+    va[r] = make_double2(c * x.x - u1.x, c * x.y - u1.y);
+    vb[r] = make_double2(u2.x + c * y.x, u2.y + c * y.y);
+  }
+}
+
+__global__ __launch_bounds__(256) void zcol_norm_kernel(const double2* __restrict__ W, int m,
+                                                        int ncols, double* __restrict__ sig) {
+  __shared__ double2 sh[4];
+  const int c = blockIdx.x;
+  if (c >= ncols) return;
+  double s = 0.0;
+  for (int r = threadIdx.x; r < m; r += 256) {
+    double2 x = W[(size_t)c * m + r];
+    s += x.x * x.x + x.y * x.y;
+  }
+  s = block_csum_256(make_double2(s, 0.0), sh).x;
+  if (threadIdx.x == 0) sig[c] = sqrt(s);
+}
+
+// kept column j: U(:, j) = W(:, perm[j]) / sigma_j (zero for sigma_j = 0),
+// V(:, j) = V0(:, perm[j]); both optionally conjugated (the transposed case)
+__global__ void zsvd_gather_kernel(int rows, int cols, int k, const double2* __restrict__ W,
+                                   const double2* __restrict__ V, const int* __restrict__ perm,
+                                   const double* __restrict__ sig, double2* __restrict__ Uo,
+                                   double2* __restrict__ Vo, int conj_out) {
+  const int64_t tot = (int64_t)std::max(rows, cols) * k;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < tot;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int j = (int)(e / std::max(rows, cols)), r = (int)(e % std::max(rows, cols));
+    const int c = perm[j];
+    const double inv = sig[j] > 0.0 ? 1.0 / sig[j] : 0.0;
+    if (r < rows) {
+      double2 x = W[(size_t)c * rows + r];
+      x = make_double2(x.x * inv, x.y * inv);
+      if (conj_out) x.y = -x.y;
+      Uo[(size_t)j * rows + r] = x;
+    }
+    if (r < cols) {
+      double2 x = V[(size_t)c * cols + r];
+      if (conj_out) x.y = -x.y;
+      Vo[(size_t)j * cols + r] = x;
+    }
+  }
+}
+
+__global__ void zconj_transpose_kernel(int m, int n, const double2* __restrict__ A,
+                                       double2* __restrict__ B) {
+  const int64_t tot = (int64_t)m * n;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < tot;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int i = (int)(e % m), c = (int)(e / m);
+    double2 x = A[e];
+    B[c + (size_t)i * n] = make_double2(x.x, -x.y);
+  }
+}
+
+}  // namespace
+
+static int zws(itg_ctx* ctx, size_t bytes, void** out) {
+  if (bytes > ctx->zla_ws_bytes) {
+    if (ctx->zla_ws) {
+      ITG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+      ITG_HIP(ctx, hipFree(ctx->zla_ws));
+      ctx->zla_ws = nullptr;
+      ctx->zla_ws_bytes = 0;
+    }
+    if (hipMalloc(&ctx->zla_ws, bytes) != hipSuccess) {
+      hipGetLastError();
+      return itg_fail(ctx, ITG_EOOM, "complex factorisation workspace of %zu bytes failed", bytes);
+    }
+    ctx->zla_ws_bytes = bytes;
+  }
+  *out = ctx->zla_ws;
+  return ITG_OK;
+}
+
+int itg_c64_qr(itg_ctx* ctx, int64_t m64, int64_t n64, const void* A, void* Qo, void* Ro) {
+  if (m64 > 8192) return itg_fail(ctx, ITG_EUNSUPPORTED, "complex qr: m = %lld > 8192",
+                                  (long long)m64);
+  const int m = (int)m64, n = (int)n64, r = std::min(m, n);
+  void* ws;
+  int rc = zws(ctx, ((size_t)m * n + r) * sizeof(double2) + 512, &ws);
+  if (rc) return rc;
+  double2* W = (double2*)ws;
+  double2* tau = W + (size_t)m * n;
+  ITG_HIP(ctx, hipMemcpyAsync(W, A, (size_t)m * n * sizeof(double2), hipMemcpyDeviceToDevice,
+                              ctx->stream));
+  for (int j = 0; j < r; ++j) {
+    zqr_reflect_kernel<<<1, 256, 0, ctx->stream>>>(W, m, j, tau);
+    if (j + 1 < n) zqr_apply_kernel<<<n - j - 1, 256, 0, ctx->stream>>>(W, m, n, j, tau);
+    ITG_LAUNCH_CHECK(ctx);
+  }
+  static bool attr = false;
+  if (!attr) {
+    ITG_HIP(ctx, hipFuncSetAttribute((const void*)zqr_form_q_kernel,
+                                     hipFuncAttributeMaxDynamicSharedMemorySize,
+                                     8192 * (int)sizeof(double2)));
+    attr = true;
+  }
+  zqr_form_q_kernel<<<r, 256, (size_t)m * sizeof(double2), ctx->stream>>>(W, m, r, tau,
+                                                                          (double2*)Qo);
+  zqr_extract_r_kernel<<<itg_blocks((int64_t)r * n, 256, 8192), 256, 0, ctx->stream>>>(
+      m, n, r, W, (double2*)Ro);
+  ITG_LAUNCH_CHECK(ctx);
+  return ITG_OK;
+}
+
+int itg_c64_svd_trunc(itg_ctx* ctx, int64_t m64, int64_t n64, void* Ain, void* Uo, double* So,
+                      void* Vo, int64_t maxdim, int64_t mindim, double cutoff, int abs_cut,
+                      int rel_cut, int64_t* kout, double* truncerr, double* docut) {
+  if (std::min(m64, n64) > 8192 || m64 * n64 > ((int64_t)1 << 30))
+    return itg_fail(ctx, ITG_EUNSUPPORTED, "complex svd: matrix too large");
+  const int m = (int)m64, n = (int)n64;
+  const bool tr = m < n;   // work on A^H so that rows >= cols
+  const int rows = tr ? n : m, cols = tr ? m : n;
+  const int N = cols + (cols & 1);
+  const size_t szW = (size_t)rows * cols, szV = (size_t)cols * cols;
+  void* ws;
+  int rc = zws(ctx, (3 * szW + 3 * szV) * sizeof(double2) + (size_t)N * 24 + 4096, &ws);
+  if (rc) return rc;
+  double2* W = (double2*)ws;
+  double2* W2 = W + szW;
+  double2* Wc = W2 + szW;             // conj(W), for the Gram product
+  double2* G = Wc + szW;
+  double2* V = G + szV;
+  double* wG = (double*)(V + szV);
+  double* offs = wG + N;
+  double* sig = offs + N;
+  int* perm = (int*)(sig + N);
+  // W = A (or A^H)
+  if (tr)
+    zconj_transpose_kernel<<<itg_blocks((int64_t)m * n, 256, 8192), 256, 0, ctx->stream>>>(
+        m, n, (const double2*)Ain, W);
+  else
+    ITG_HIP(ctx, hipMemcpyAsync(W, Ain, szW * sizeof(double2), hipMemcpyDeviceToDevice,
+                                ctx->stream));
+  ITG_LAUNCH_CHECK(ctx);
+  // preconditioning: G = W^H W = V0 diag V0^H, W <- W V0, V <- V0
+  ITG_HIP(ctx, hipMemcpyAsync(Wc, W, szW * sizeof(double2), hipMemcpyDeviceToDevice,
+                              ctx->stream));
+  rc = itg_conj(ctx, ITG_C64, (int64_t)szW, Wc);
+  if (rc) return rc;
+  {
+    const int64_t dW[2] = {rows, cols}, dG[2] = {cols, cols};
+    const int32_t lWc[2] = {1, 2}, lW[2] = {1, 3}, lG[2] = {2, 3};
+    rc = itg_contract(ctx, ITG_C64, ITG_MODE_NATIVE, 2, dW, lWc, Wc, 2, dW, lW, W, 2, dG, lG, G,
+                      1.0, 0.0);
+    if (rc) return rc;
+    int64_t kg = 0;
+    double teg = 0.0, dcg = 0.0;
+    rc = itg_eigh_trunc(ctx, ITG_C64, cols, G, wG, V, 0, -1.0, 0, 1, &kg, &teg, &dcg);
+    if (rc) return rc;
+    const int32_t lA[2] = {1, 2}, lV[2] = {2, 3}, lO[2] = {1, 3};
+    rc = itg_contract(ctx, ITG_C64, ITG_MODE_NATIVE, 2, dW, lA, W, 2, dG, lV, V, 2, dW, lO, W2,
+                      1.0, 0.0);
+    if (rc) return rc;
+  }
+  W = W2;
+  const double tol = 4.0 * 1.1102230246251565e-16 * std::sqrt((double)rows);
+  double fro = 0.0;
+  rc = itg_nrm2(ctx, ITG_C64, (int64_t)szW, W, &fro);
+  if (rc) return rc;
+  const double small2 = (1.1102230246251565e-16 * fro) * (1.1102230246251565e-16 * fro);
+  std::vector<double> hoffs(N / 2);
+  bool conv = cols < 2;
+  for (int sweep = 0; sweep < 60 && !conv; ++sweep) {
+    ITG_HIP(ctx, hipMemsetAsync(offs, 0, (N / 2) * sizeof(double), ctx->stream));
+    for (int rnd = 0; rnd < N - 1; ++rnd)
+      zjac_round_kernel<<<N / 2, 256, 0, ctx->stream>>>(W, rows, V, cols, cols, N, rnd, tol,
+                                                        small2, offs);
+    ITG_LAUNCH_CHECK(ctx);
+    ITG_HIP(ctx, hipMemcpyAsync(hoffs.data(), offs, (N / 2) * sizeof(double),
+                                hipMemcpyDeviceToHost, ctx->stream));
+    ITG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    double mx = 0.0;
+    for (double x : hoffs) mx = std::max(mx, x);
+    conv = mx <= tol;
+  }
+  if (!conv) return itg_fail(ctx, ITG_ECONV, "complex svd: Jacobi sweeps did not converge");
+  zcol_norm_kernel<<<cols, 256, 0, ctx->stream>>>(W, rows, cols, wG);
+  ITG_LAUNCH_CHECK(ctx);
+  // descending order (on the host: one read-back of cols values)
+  std::vector<double> hs(cols);
+  ITG_HIP(ctx, hipMemcpyAsync(hs.data(), wG, cols * sizeof(double), hipMemcpyDeviceToHost,
+                              ctx->stream));
+  ITG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  std::vector<int> hp(cols);
+  for (int i = 0; i < cols; ++i) hp[i] = i;
+  std::stable_sort(hp.begin(), hp.end(), [&](int a, int b) { return hs[a] > hs[b]; });
+  std::vector<double> sorted(cols), P(cols);
+  for (int i = 0; i < cols; ++i) {
+    sorted[i] = hs[hp[i]];
+    P[i] = sorted[i] * sorted[i];
+  }
+  int64_t k = 0;
+  double te = 0.0, dc = 0.0;
+  rc = itg_truncate(ctx, P.data(), cols, maxdim, mindim, cutoff, abs_cut, rel_cut,
+                    ITG_TRUNC_NDTENSORS, &k, &te, &dc);
+  if (rc) return rc;
+  ITG_HIP(ctx, hipMemcpyAsync(sig, sorted.data(), cols * sizeof(double), hipMemcpyHostToDevice,
+                              ctx->stream));
+  ITG_HIP(ctx, hipMemcpyAsync(perm, hp.data(), cols * sizeof(int), hipMemcpyHostToDevice,
+                              ctx->stream));
+  // A = U S V^H; for A^H = U' S V'^H, A = V' S U'^H: U = V', V = U'
+  double2* Ua = (double2*)(tr ? Vo : Uo);
+  double2* Va = (double2*)(tr ? Uo : Vo);
+  zsvd_gather_kernel<<<itg_blocks((int64_t)std::max(rows, cols) * k, 256, 8192), 256, 0,
+                       ctx->stream>>>(rows, cols, (int)k, W, V, perm, sig, Ua, Va, 0);
+  ITG_LAUNCH_CHECK(ctx);
+  ITG_HIP(ctx, hipMemcpyAsync(So, sig, std::min(m, n) * sizeof(double), hipMemcpyDeviceToDevice,
+                              ctx->stream));
+  ITG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  *kout = k;
+  if (truncerr) *truncerr = te;
+  if (docut) *docut = dc;
+  return ITG_OK;
+}

@@ -77,6 +77,7 @@ extern "C" int itg_destroy(itg_ctx* ctx) {
   if (ctx->mlp_ws) hipFree(ctx->mlp_ws);
   if (ctx->cws) hipFree(ctx->cws);
   if (ctx->svd_ws) hipFree(ctx->svd_ws);
+  if (ctx->zla_ws) hipFree(ctx->zla_ws);
   if (ctx->red) hipFree(ctx->red);
   if (ctx->red_host) hipHostFree(ctx->red_host);
   hipStreamDestroy(ctx->own_stream);

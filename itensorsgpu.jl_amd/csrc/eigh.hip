@@ -1629,10 +1629,11 @@ __global__ void transpose_kernel(int m, int n, const double* __restrict__ A, dou
 extern "C" int itg_qr(itg_ctx* ctx, int dtype, int64_t m64, int64_t n64, const void* Ain, void* Qo,
                       void* Ro) {
   ITG_CHECK_CTX(ctx);
-  ITG_CHECK_DTYPE(ctx, dtype);
+  if (dtype != ITG_C64) ITG_CHECK_DTYPE(ctx, dtype);
   if (m64 < 0 || n64 < 0) return itg_fail(ctx, ITG_EDIM, "qr: negative dimension");
   if (m64 * n64 > ((int64_t)1 << 31)) return itg_fail(ctx, ITG_EUNSUPPORTED, "qr: matrix too large");
   if (m64 == 0 || n64 == 0) return ITG_OK;
+  if (dtype == ITG_C64) return itg_c64_qr(ctx, m64, n64, Ain, Qo, Ro);
   int m = (int)m64, n = (int)n64, r = std::min(m, n);
   bool f32 = dtype == ITG_F32;
   size_t bytes = ((size_t)m * n + (size_t)m * r + (size_t)r * n + (size_t)m * NB + 2 * (size_t)NB * n +
@@ -1706,7 +1707,7 @@ extern "C" int itg_svd_trunc(itg_ctx* ctx, int dtype, int64_t m64, int64_t n64, 
                              int abs_cut, int rel_cut, int64_t* kout, double* truncerr,
                              double* docut) {
   ITG_CHECK_CTX(ctx);
-  ITG_CHECK_DTYPE(ctx, dtype);
+  if (dtype != ITG_C64) ITG_CHECK_DTYPE(ctx, dtype);
   if (m64 < 0 || n64 < 0) return itg_fail(ctx, ITG_EDIM, "svd: negative dimension");
   if (!kout) return itg_fail(ctx, ITG_EARG, "svd: null k");
   if (m64 == 0 || n64 == 0) {
@@ -1715,6 +1716,9 @@ extern "C" int itg_svd_trunc(itg_ctx* ctx, int dtype, int64_t m64, int64_t n64, 
     if (docut) *docut = 0.0;
     return ITG_OK;
   }
+  if (dtype == ITG_C64)
+    return itg_c64_svd_trunc(ctx, m64, n64, Ain, Uo, So, Vo, maxdim, mindim, cutoff, abs_cut,
+                             rel_cut, kout, truncerr, docut);
   if (std::min(m64, n64) > 8192 || m64 * n64 > ((int64_t)1 << 31))
     return itg_fail(ctx, ITG_EUNSUPPORTED, "svd: matrix too large");
   int rc = set_lds_limit(ctx);

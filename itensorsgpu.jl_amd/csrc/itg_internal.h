@@ -59,6 +59,9 @@ struct itg_ctx {
   // the eigensolver)
   void* svd_ws = nullptr;
   size_t svd_ws_bytes = 0;
+  // grow-only device workspace of the complex QR and SVD
+  void* zla_ws = nullptr;
+  size_t zla_ws_bytes = 0;
   // grow-only device workspace of the complex contraction's expanded operand
   void* cws = nullptr;
   size_t cws_bytes = 0;
@@ -155,3 +158,7 @@ int itg_zhetrd(itg_ctx* ctx, int n, void* A, double* d, double* e, void* tau, do
 // (n x k, leading dimension n)
 int itg_zunmtr(itg_ctx* ctx, int n, int k, const void* A, const void* tau, const double* Z,
                void* U);
+int itg_c64_qr(itg_ctx* ctx, int64_t m, int64_t n, const void* A, void* Q, void* R);
+int itg_c64_svd_trunc(itg_ctx* ctx, int64_t m, int64_t n, void* A, void* U, double* S, void* V,
+                      int64_t maxdim, int64_t mindim, double cutoff, int abs_cut, int rel_cut,
+                      int64_t* k, double* truncerr, double* docut);

@@ -642,12 +642,29 @@ def noiseterm(PH, b, phi, pdims, ortho, mode="native"):
     return ops.add_permuted(drho, (m, m), (0, 1), 0.5, sym, 1.0)
 
 
-def _replacebond_complex(phi, m, n, ortho, maxdim, cutoff, drho, noise):
-    """Complex phi (m x n): left, rho = phi phi^H = U D U^H, A = U, B = U^H phi;
-    right, rho = phi^H phi = V D V^H, B = V^H, A = phi V (B, A normalised as
-    the real route does)."""
+def _replacebond_complex(phi, m, n, ortho, maxdim, mindim, cutoff, drho, noise, use_eigen):
+    """Complex phi (m x n). Density-matrix route: left, rho = phi phi^H =
+    U D U^H, A = U, B = U^H phi; right, rho = phi^H phi = V D V^H, B = V^H,
+    A = phi V. SVD route (phi = U S V^H): left A = U, B = S V^H; right
+    A = U S, B = V^H. The non-orthogonal factor is normalised, as in the
+    real routes."""
     cut = 0.0 if cutoff is None else cutoff
     pb = _bra(phi)
+    if not use_eigen:
+        U, S, V, k, terr, _ = ops.svd_trunc(ops.copy(phi), m, n, maxdim=maxdim, mindim=mindim,
+                                            cutoff=cut)
+        s = ops.to_host(S)[:k]
+        Sd = ops.to_device(np.ravel(np.diag(s).astype(np.complex128), order="F"), "complex128")
+        Vh, _ = ops.permute(_bra(V[: n * k]), (n, k), (1, 0))          # k x n
+        if ortho == "left":
+            A = U[: m * k].clone()
+            B, _ = ops.contract(Sd, (k, k), [1, 2], Vh, (k, n), [2, 3], [1, 3])
+            ops.scal(B, 1.0 / ops.nrm2(B))
+        else:
+            B = Vh
+            A, _ = ops.contract(U[: m * k], (m, k), [1, 2], Sd, (k, k), [2, 3], [1, 3])
+            ops.scal(A, 1.0 / ops.nrm2(A))
+        return A, B, k, terr
     if ortho == "left":
         rho, _ = ops.contract(phi, (m, n), [1, 2], pb, (m, n), [3, 2], [1, 3])
         if drho is not None:
@@ -678,9 +695,8 @@ def replacebond(psi, b, phi, pdims, ortho, maxdim, mindim, cutoff, drho=None, no
     use_eigen = (cutoff is not None and cutoff >= 1e-12) or drho is not None
     dt = str(phi.dtype).split(".")[-1]
     if phi.is_complex():
-        # the density-matrix route on ComplexF64 (heevd); there is no
-        # complex SVD, so it is taken at every cutoff
-        A, B, k, terr = _replacebond_complex(phi, m, n, ortho, maxdim, cutoff, drho, noise)
+        A, B, k, terr = _replacebond_complex(phi, m, n, ortho, maxdim, mindim, cutoff, drho,
+                                             noise, use_eigen)
     elif use_eigen:
         if ortho == "left":
             rho = ops.empty(m * m, dt)

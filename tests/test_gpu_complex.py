@@ -300,3 +300,78 @@ def test_complex_dmrg_dm_interaction_vs_ed(dev):
     assert psi.is_complex()
     assert abs(E - E0) < 1e-9
     assert abs(expect_energy(H, psi) - E) < 1e-10
+
+
+# ----------------------------------------------------------------- QR, SVD
+@pytest.mark.parametrize("m,n", [(20, 12), (12, 20), (64, 64), (300, 130), (1, 1)])
+def test_qr_complex(dev, m, n):
+    """Complex thin QR (the reference's qr over ComplexF64 storage): A = QR,
+    Q^H Q = I, R upper triangular with LAPACK's |diag R|."""
+    rng = np.random.default_rng(m * 7 + n)
+    A = crandn(rng, m, n)
+    Q, R = dev.qr(up(dev, A), m, n)
+    r = min(m, n)
+    Qh = ops.from_flat(dev.to_host(Q), (m, r))
+    Rh = ops.from_flat(dev.to_host(R), (r, n))
+    np.testing.assert_allclose(Qh @ Rh, A, atol=1e-13 * max(1.0, np.abs(A).max()))
+    assert np.abs(Qh.conj().T @ Qh - np.eye(r)).max() < 1e-13
+    assert np.abs(np.tril(Rh, -1)).max() == 0.0
+    np.testing.assert_allclose(np.abs(np.diag(Rh)), np.abs(np.diag(np.linalg.qr(A)[1])),
+                               rtol=1e-12)
+
+
+@pytest.mark.parametrize("m,n", [(50, 30), (30, 50), (200, 200), (300, 120)])
+def test_svd_complex(dev, m, n):
+    """Complex truncated SVD (A = U S V^H): singular values against LAPACK's
+    zgesdd, unitary factors, reconstruction; then a graded spectrum
+    truncated by cutoff as truncate! decides on LAPACK's values."""
+    from oracle.truncate import ndtensors_truncate
+    rng = np.random.default_rng(m + 3 * n)
+    A = crandn(rng, m, n)
+    U, S, V, k, te, dc = dev.svd_trunc(up(dev, A), m, n)
+    r = min(m, n)
+    assert k == r
+    Uh = ops.from_flat(dev.to_host(U)[: m * k], (m, k))
+    Vh = ops.from_flat(dev.to_host(V)[: n * k], (n, k))
+    Sh = dev.to_host(S)[:k]
+    np.testing.assert_allclose(Sh, np.linalg.svd(A, compute_uv=False), rtol=1e-12)
+    assert np.abs(Uh.conj().T @ Uh - np.eye(k)).max() < 1e-12
+    assert np.abs(Vh.conj().T @ Vh - np.eye(k)).max() < 1e-12
+    np.testing.assert_allclose((Uh * Sh) @ Vh.conj().T, A, atol=1e-12 * np.abs(A).max())
+    U0, _ = np.linalg.qr(crandn(rng, m, r))
+    V0, _ = np.linalg.qr(crandn(rng, n, r))
+    s = 10.0 ** -np.linspace(0, 8, r)
+    B = (U0 * s) @ V0.conj().T
+    U, S, V, k, te, dc = dev.svd_trunc(up(dev, B), m, n, maxdim=r - 3, cutoff=1e-12)
+    kr, ter, _ = ndtensors_truncate(s ** 2, maxdim=r - 3, cutoff=1e-12)
+    assert k == kr
+    assert abs(te - ter) <= 1e-6 * ter + 1e-24
+    np.testing.assert_allclose(dev.to_host(S)[:k], s[:k], rtol=1e-9)
+
+
+def test_complex_mps_orthogonalize_and_svd_route(dev):
+    """A complex random MPS orthogonalised by complex QR keeps its norm and
+    inner products; complex DMRG through the SVD route (cutoff below 1e-12)
+    reaches the same energy as the density-matrix route."""
+    from itg import mpo
+    from itg.itensor import Index
+    from itg.mps import MPO, OpSum, Sweeps, dmrg, random_mps
+    sites = [Index(2, "Site,n=%d" % j) for j in range(6)]
+    psi = mpo.random_mps(sites, linkdims=4, seed=3, dtype="complex128")
+    assert psi[0].is_complex()
+    assert abs(mpo.inner(psi, psi) - 1.0) < 1e-12
+    phi = psi.copy()
+    mpo.orthogonalize(phi, 3)
+    assert abs(mpo.inner(psi, phi) - 1.0) < 1e-12
+    N, D = 8, 0.4
+    os_ = OpSum()
+    for j in range(N - 1):
+        os_.add(1.0, "Sz", j, "Sz", j + 1)
+        os_.add(0.5, "S+", j, "S-", j + 1)
+        os_.add(0.5, "S-", j, "S+", j + 1)
+        os_.add(0.5j * D, "S+", j, "S-", j + 1)
+        os_.add(-0.5j * D, "S-", j, "S+", j + 1)
+    H = MPO(os_.to_mpo_arrays(N))
+    E_eig = dmrg(H, random_mps(N, 2, 4, seed=5), Sweeps(6, maxdim=[8, 16, 32], cutoff=[1e-12]))
+    E_svd = dmrg(H, random_mps(N, 2, 4, seed=5), Sweeps(6, maxdim=[8, 16, 32], cutoff=[1e-14]))
+    assert abs(E_eig - E_svd) < 1e-9
