@@ -6,6 +6,7 @@
 // leading dimension ldk (a multiple of 64, zero padded for k >= K), so an
 // MFMA lane's 8 consecutive k are one 16-byte load and the GEMM's inner loop
 // does no conversion and reads 4 bytes per element instead of 8.
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -254,17 +255,19 @@ int itg_gemm_split(itg_ctx* ctx, int dtype, int64_t M, int64_t N, int64_t K, int
   int64_t tilesM = (M + SB_M - 1) / SB_M, tilesN = (N + SB_N - 1) / SB_N;
   int64_t tiles = tilesM * tilesN;
   if (tiles > 0x7fffffff) return itg_fail(ctx, ITG_EUNSUPPORTED, "split gemm: too large");
-  int tk = itg_tstart(ctx, ITG_K_GEMM_SPLIT);
+  itg_tev tk = itg_tevents(ctx, ITG_K_GEMM_SPLIT);
   if (dtype == ITG_F64)
-    gemm_split_kernel<double><<<(unsigned)tiles, 256, lds, ctx->stream>>>(
-        Ah, Al, lda, Bh, Bl, ldb, M, N, Kp, (double*)C, ldc, alpha, beta, tilesM, tilesN);
+    hipExtLaunchKernelGGL(gemm_split_kernel<double>, dim3((unsigned)tiles), dim3(256),
+                          (uint32_t)lds, ctx->stream, tk.a, tk.b, 0, Ah, Al, lda, Bh, Bl, ldb, M,
+                          N, Kp, (double*)C, ldc, alpha, beta, tilesM, tilesN);
   else
-    gemm_split_kernel<float><<<(unsigned)tiles, 256, lds, ctx->stream>>>(
-        Ah, Al, lda, Bh, Bl, ldb, M, N, Kp, (float*)C, ldc, alpha, beta, tilesM, tilesN);
+    hipExtLaunchKernelGGL(gemm_split_kernel<float>, dim3((unsigned)tiles), dim3(256),
+                          (uint32_t)lds, ctx->stream, tk.a, tk.b, 0, Ah, Al, lda, Bh, Bl, ldb, M,
+                          N, Kp, (float*)C, ldc, alpha, beta, tilesM, tilesN);
   ITG_LAUNCH_CHECK(ctx);
   // algorithmic: 2MNK flops; split planes read once, C written (and read)
   double es = (double)itg_dtype_size(dtype);
-  itg_tstop(ctx, tk, ITG_K_GEMM_SPLIT, 2.0 * M * N * K,
+  itg_tcommit(ctx, tk, ITG_K_GEMM_SPLIT, 2.0 * M * N * K,
             4.0 * (M + N) * K + es * M * N * (beta != 0.0 ? 2.0 : 1.0));
   return ITG_OK;
 }

@@ -127,6 +127,26 @@ int itg_tstart(itg_ctx* ctx, int kind) {
   return (int)ctx->launches.size() - 1;
 }
 
+itg_tev itg_tevents(itg_ctx* ctx, int kind) {
+  itg_tev ev = {-1, nullptr, nullptr};
+  if (!(ctx->timing & (1 << kind))) return ev;
+  hipEvent_t a = timing_event(ctx), b = timing_event(ctx);
+  if (!a || !b) return ev;
+  ctx->launches.push_back({-1, a, b, 0.0, 0.0});
+  ev.t = (int)ctx->launches.size() - 1;
+  ev.a = a;
+  ev.b = b;
+  return ev;
+}
+
+void itg_tcommit(itg_ctx* ctx, const itg_tev& ev, int kind, double flops, double bytes) {
+  if (ev.t < 0) return;
+  auto& L = ctx->launches[ev.t];
+  L.kind = kind;
+  L.flops = flops;
+  L.bytes = bytes;
+}
+
 void itg_tstop(itg_ctx* ctx, int t, int kind, double flops, double bytes) {
   if (t < 0) return;
   auto& L = ctx->launches[t];

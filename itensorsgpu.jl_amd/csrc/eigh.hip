@@ -16,6 +16,7 @@
 // 3. Eigenvalues sorted descending, truncate! decided on the device, and only
 //    the k kept eigenvectors back-transformed by the compact-WY block
 //    reflectors (Schreiber & Van Loan 1989) on MFMA.
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -1118,25 +1119,28 @@ static int tridiagonalize(itg_ctx* ctx, int n, double* A, double* d, double* e, 
       int m = n - j - 1;
       int T = (m + TS - 1) / TS;
       int ntiles = T * (T + 1) / 2;
-      int tk = itg_tstart(ctx, ITG_K_TRD_SYMV);
-      trd_symv_tiles_kernel<<<ntiles + i, 256, 0, ctx->stream>>>(A, n, j, i, pnorm + pn_off,
-                                                                 pn_cnt, Vp, Wp, V2, Ypart, ab, pyv,
-                                                                 ntiles, d, e, tau);
+      itg_tev tk = itg_tevents(ctx, ITG_K_TRD_SYMV);
+      hipExtLaunchKernelGGL(trd_symv_tiles_kernel, dim3(ntiles + i), dim3(256), 0, ctx->stream,
+                            tk.a, tk.b, 0, (const double*)A, n, j, i,
+                            (const double*)(pnorm + pn_off), pn_cnt, Vp, (const double*)Wp, V2,
+                            Ypart, ab, pyv, ntiles, d, e, tau);
       ITG_LAUNCH_CHECK(ctx);
       // lower triangle of A22 read once, the panel pairs over m rows
-      itg_tstop(ctx, tk, ITG_K_TRD_SYMV, 2.0 * m * (m + 2.0 * i),
-                8.0 * (0.5 * m * (m + TS) + 2.0 * i * m + 2.0 * T * m));
+      itg_tcommit(ctx, tk, ITG_K_TRD_SYMV, 2.0 * m * (m + 2.0 * i),
+                  8.0 * (0.5 * m * (m + TS) + 2.0 * i * m + 2.0 * T * m));
       int next = (i + 1 < nbp) ? 1 : 0;
       const int b0 = (j + 1) / POST_ROWS;
       const int nblk = (n + POST_ROWS - 1) / POST_ROWS - b0;
-      tk = itg_tstart(ctx, ITG_K_TRD_POST);
-      trd_post_kernel<<<nblk, 256, 0, ctx->stream>>>(A, n, j, i, ntiles, Vp, Wp, Ypart, T, ab, pyv,
-                                                     tau, next, pnorm, b0);
+      itg_tev tp = itg_tevents(ctx, ITG_K_TRD_POST);
+      hipExtLaunchKernelGGL(trd_post_kernel, dim3(nblk), dim3(256), 0, ctx->stream, tp.a, tp.b, 0,
+                            A, n, j, i, ntiles, (const double*)Vp, Wp, (const double*)Ypart, T,
+                            (const double*)ab, (const double*)pyv, (const double*)tau, next,
+                            pnorm, b0);
       ITG_LAUNCH_CHECK(ctx);
       // the next column's reflector norm: partials of these blocks only
       pn_off = b0;
       pn_cnt = nblk;
-      itg_tstop(ctx, tk, ITG_K_TRD_POST, 8.0 * m * (i + 1), 8.0 * m * (2.0 * i + 5 + T));
+      itg_tcommit(ctx, tp, ITG_K_TRD_POST, 8.0 * m * (i + 1), 8.0 * m * (2.0 * i + 5 + T));
     }
     int q = p + nbp;
     if (q < n) {

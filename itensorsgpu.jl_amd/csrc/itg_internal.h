@@ -130,6 +130,16 @@ void itg_plan_cache_clear(itg_ctx* ctx);
 int itg_tstart(itg_ctx* ctx, int kind);
 void itg_tstop(itg_ctx* ctx, int t, int kind, double flops, double bytes);
 void itg_timing_free(itg_ctx* ctx);
+// For a launch made with hipExtLaunchKernelGGL(..., ev.a, ev.b, 0, ...): the
+// runtime stamps the two events at the kernel's own start and end, so short
+// kernels are timed without the dispatch gap a separate event record adds.
+// ev.a/ev.b are null when the kind is not timed (a plain launch).
+struct itg_tev {
+  int t;
+  hipEvent_t a, b;
+};
+itg_tev itg_tevents(itg_ctx* ctx, int kind);
+void itg_tcommit(itg_ctx* ctx, const itg_tev& ev, int kind, double flops, double bytes);
 void itg_graph_cache_clear(itg_ctx* ctx);
 // Resolves (once per device, with that device current) the device address of
 // the no-alloc scratch array into ctx->noalloc_scratch.
