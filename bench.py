@@ -239,12 +239,14 @@ def precision_check(N, chi, nsweep, krylov, modes, budget, reserve):
     return out, recs
 
 
-def kernel_sample(chi, reps=3):
+def kernel_sample(chi, reps=3, eigh_n=0):
     """Short fixed workload for hardware-counter passes: an in-place scale of
     2^27 doubles (exactly 1 GiB read and 1 GiB written: the calibration of
     FETCH_SIZE/WRITE_SIZE), then `reps` prepared split-BF16 Heff applies at
-    the bulk-bond shape (chi, w=5, d=2). Counter passes serialise every
-    dispatch, so the sample stays at a few dozen launches."""
+    the bulk-bond shape (chi, w=5, d=2), and with eigh_n > 0 one
+    density-matrix eigendecomposition of that size (maxdim n/2). Counter
+    passes serialise every dispatch, so the Heff sample stays at a few dozen
+    launches; the eigendecomposition adds two per tridiagonalised column."""
     import torch
     from itg import ops
     x = ops.randn(1 << 27, 1)
@@ -265,6 +267,14 @@ def kernel_sample(chi, reps=3):
     for _ in range(reps):
         ops.heff2_apply_prepared(L, R, prep, psi, *dims, mode="bf16x3", out=y, work=work)
     torch.cuda.synchronize()
+    if eigh_n:
+        del L, R, psi, y, work, prep
+        A = ops.randn(eigh_n * eigh_n, 3)
+        rho = ops.empty(eigh_n * eigh_n)
+        ops.gemm("N", "T", eigh_n, eigh_n, eigh_n, 1.0 / eigh_n, A, eigh_n, A, eigh_n, 0.0, rho,
+                 eigh_n)
+        ops.eigh_trunc(rho, eigh_n, maxdim=eigh_n // 2, cutoff=1e-12)
+        torch.cuda.synchronize()
     log("kernel sample done")
 
 
@@ -485,6 +495,8 @@ def main():
     ap.add_argument("--ab-sweeps", type=int, default=3)
     ap.add_argument("--kernel-sample", action="store_true",
                     help="short fixed workload for rocprofv3 --pmc passes (see kernel_sample)")
+    ap.add_argument("--sample-eigh", type=int, default=0,
+                    help="with --kernel-sample: one eigendecomposition of this size after it")
     ap.add_argument("--config3", action="store_true",
                     help="config-3 start: link dims 10, ramp 64..1024, cutoff 1e-12")
     ap.add_argument("--cpu-chi", type=int, default=0,
@@ -510,7 +522,7 @@ def main():
         print(json.dumps({"micro": {c: micro(chi=c) for c in (1024, 2048)}, "device": info}))
         return
     if args.kernel_sample:
-        kernel_sample(args.chi)
+        kernel_sample(args.chi, eigh_n=args.sample_eigh)
         return
     if args.eigh:
         print(json.dumps({"eigh": eigh_only(args.eigh), "device": info}))
