@@ -156,8 +156,13 @@ __global__ __launch_bounds__(256) void trd_symv_tiles_kernel(
       araw = l < m ? a[o + l] : 0.0;
     }
   }
+  // loads that do not wait for the reflector go out before its reduction
+  // (a barrier would otherwise order them after it)
+  const double alpha = a[j + 1];
+  const int rv0 = blockIdx.x * 256 + t;
+  const double av0 = (rv0 < n && rv0 > j + 1) ? a[rv0] : 0.0;
+  const double ajj = (blockIdx.x == 0 && t == 0) ? a[j] : 0.0;
   double xn2 = sum_partials(pnorm, npn, sh4);
-  double alpha = a[j + 1];
   double beta = alpha, tv = 0.0, scale = 1.0;
   if (xn2 != 0.0) {
     beta = -copysign(sqrt(alpha * alpha + xn2), alpha);
@@ -165,12 +170,12 @@ __global__ __launch_bounds__(256) void trd_symv_tiles_kernel(
     scale = 1.0 / (alpha - beta);
   }
   if (blockIdx.x == 0 && t == 0) {
-    d[j] = a[j];
+    d[j] = ajj;
     e[j] = beta;
     tau[j] = tv;
   }
-  for (int r = blockIdx.x * 256 + t; r < n; r += gridDim.x * 256) {
-    double v = r <= j ? 0.0 : (r == j + 1 ? 1.0 : a[r] * scale);
+  for (int r = rv0; r < n; r += gridDim.x * 256) {
+    double v = r <= j ? 0.0 : (r == j + 1 ? 1.0 : (r == rv0 ? av0 : a[r]) * scale);
     Vp[r + (size_t)i * ld] = v;
     V2[r + (size_t)i * ld] = v;
   }
@@ -311,6 +316,13 @@ __global__ __launch_bounds__(256) void trd_post_kernel(
     if (next && lane < i)
       s1 = Vp[jn + (size_t)lane * ld] * ab[lane] + Wp[jn + (size_t)lane * ld] * ab[i + lane];
   }
+  // wave 0's operands of the final step, loaded before the barrier
+  const double tj = tau[j];
+  double vr0 = 0.0, ajn0 = 0.0;
+  if (wv == 0 && lane < POST_ROWS && r < n) {
+    vr0 = Vp[r + (size_t)i * ld];
+    if (next && r >= jn) ajn0 = A[r + (size_t)jn * ld];
+  }
   pv = wave_sum(pv);
   s1 = wave_sum(s1);
   red[0][part][rr] = yp;
@@ -322,7 +334,6 @@ __global__ __launch_bounds__(256) void trd_post_kernel(
   }
   __syncthreads();
   if (wv != 0) return;
-  const double tj = tau[j];
   const double ytv = (sc[0][0] + sc[0][1]) + (sc[0][2] + sc[0][3]);
   const double y0 = sc[1][1], atb = sc[1][2], djn = sc[1][3];
   const double alpha = -0.5 * tj * tj * (ytv - 2.0 * atb);
@@ -335,7 +346,7 @@ __global__ __launch_bounds__(256) void trd_post_kernel(
       swr += red[1][p][lane];
       snr += red[2][p][lane];
     }
-    const double vr = Vp[r + (size_t)i * ld];
+    const double vr = vr0;
     double wr = 0.0;
     if (r > j && tj != 0.0) wr = tj * (yr - swr) + alpha * vr;
     Wp[r + (size_t)i * ld] = wr;
@@ -343,7 +354,7 @@ __global__ __launch_bounds__(256) void trd_post_kernel(
     if (next && r >= jn) {
       // w(jn) for the panel column i (V(jn, i) = 1)
       const double wj1 = (tj != 0.0) ? tj * (y0 - djn) + alpha : 0.0;
-      double acc = A[r + (size_t)jn * ld] - snr;
+      double acc = ajn0 - snr;
       acc -= vr * wj1 + wr;
       A[r + (size_t)jn * ld] = acc;
       if (r >= jn + 2) s = acc * acc;
