@@ -354,9 +354,37 @@ def learned_form(N, chi, krylov, train_chi=64, steps_per_bond=10):
     _lib.ctx().sync()
     t2 = time.perf_counter()
     return {"s_per_sweep": t2 - t1, "energy": E, "matvecs": rec[0]["matvecs"],
+            "matvecs_per_bond": rec[0]["matvecs"] / (2.0 * (N - 1)),
+            "maxerr": rec[0]["maxerr"],
             "reverted": rec[0]["reverted"], "phases": timer.report(),
             "train": {"seconds": train_s, "adam_steps": net.step, "chi": train_chi,
                       "objective": "one-step Ritz value"}}
+
+
+def diag_form(N, chi, krylov):
+    """One split-BF16 sweep from the same random MPS (seed 1) with the
+    diagonal-preconditioned device Davidson: the classical preconditioner
+    beside the plain and learned forms."""
+    import torch
+    from itg import _lib
+    from itg.mps import MPO, PhaseTimer, ProjMPO, Sweeps, dmrg, heisenberg_opsum, random_mps
+    H = MPO.from_opsum(heisenberg_opsum([(j, j + 1) for j in range(N - 1)]), N)
+    psi = random_mps(N, 2, chi, seed=1)
+    ph = ProjMPO(H, "bf16x3")
+    ph.build_right(psi)
+    rec = []
+    torch.cuda.synchronize()
+    _lib.ctx().sync()
+    t1 = time.perf_counter()
+    E = dmrg(H, psi, Sweeps(1, maxdim=[chi], cutoff=[1e-12]), mode="bf16x3", krylovdim=krylov,
+             ph=ph, record=rec, precond="diag",
+             sweep_callback=lambda r: log("[diag] timed", r))
+    torch.cuda.synchronize()
+    _lib.ctx().sync()
+    t2 = time.perf_counter()
+    return {"s_per_sweep": t2 - t1, "energy": E, "matvecs": rec[0]["matvecs"],
+            "matvecs_per_bond": rec[0]["matvecs"] / (2.0 * (N - 1)),
+            "maxerr": rec[0]["maxerr"]}
 
 
 def cut_dims(N, d, chi):
@@ -600,6 +628,13 @@ def main():
         else:
             notes.append("learned form skipped: %.0f s left, ~%.0f s needed"
                          % (budget.left(), est))
+    diag = None
+    if learned is not None:
+        if budget.left() > r["s_per_sweep"] + 25:
+            diag = diag_form(args.N, args.chi, args.krylov)
+            log("diag form", diag)
+        else:
+            notes.append("diagonal-preconditioner form skipped: %.0f s left" % budget.left())
     eig = None
     if budget.left() > 15:
         eig = eigensolver_kernels(2 * args.chi)
@@ -696,8 +731,12 @@ def main():
                 "energy": pc, "sweeps": pc_rec}
         elif pc_rec is not None:
             line["precision_check"] = {"incomplete": pc_rec}
+    if diag is not None:
+        line["diag_form"] = diag
     if learned is not None:
         learned["energy_plain_sweep"] = r["energy"]
+        learned["plain_matvecs_per_bond"] = r["sweeps"][0]["matvecs"] / (2.0 * (args.N - 1)) \
+            if r.get("sweeps") else None
         if len(modes) > 1:
             learned["speedup_vs_" + modes[1]] = res[modes[1]]["s_per_sweep"] / learned["s_per_sweep"]
         line["learned_form"] = learned
