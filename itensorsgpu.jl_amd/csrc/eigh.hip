@@ -54,13 +54,14 @@ __device__ double block_sum(double v, double* sh) {
 
 // ------------------------------------------------------- tridiagonalisation
 // Per column j (panel column i) two kernels run across the whole GPU:
-//  trd_symv_tiles: every block forms the reflector of column j from the
-//    norm partials left by the previous kernel (identically in every block);
-//    one block per 64 x 64 tile on or below the diagonal of A22 leaves row
-//    and column partials of y = A22 v (one writer per slot) and a partial
-//    of v^T A22 v; extra blocks form the panel dots a = W^T v and
-//    b = V^T v; the grid stores v into the panel.
-//  trd_post: w = tau (y - V a - W b) + alpha v, where
+//  trd_symv: one block per 64 x 64 tile on or below the diagonal of A22
+//    leaves row and column partials of A22 a (a = column j below the
+//    diagonal, one writer per slot) and a partial of a^T A22 a; these need
+//    neither the reflector's norm nor its scale. Extra blocks form the
+//    reflector from the norm partials left by the previous kernel and the
+//    panel dots a = W^T v and b = V^T v; the grid stores v into the panel.
+//  trd_post: y = A22 v = s (A22 a - beta A22(:, 0)), then
+//    w = tau (y - V a - W b) + alpha v, where
 //    alpha = -tau/2 w^T v = -tau^2/2 (y^T v - 2 a^T b) needs no extra
 //    reduction pass; v goes into A; then the next panel column's update
 //    A(:,j+1) -= V W(j+1,:)^T + W V(j+1,:)^T and the partial sums of its
@@ -103,24 +104,39 @@ __global__ __launch_bounds__(256) void trd_colnorm_kernel(const double* __restri
 }
 
 // Symmetric product of the trailing matrix A22 (rows/cols o = j+1 .. n-1,
-// local index l = r - o, size m) with the reflector v, reading only the
-// 64 x 64 tiles on and below the diagonal (both triangles of a diagonal tile
-// are stored, so it is used whole). Tile (I, J), I >= J, writes its row
-// partial sum_{c in J} A[r,c] v[c] to Ypart[J][r] and, below the diagonal,
-// its column partial sum_{r in I} A[r,c] v[r] to Ypart[I][c]: every (t, l)
-// of Ypart (t < T tile rows) has exactly one writer and y(l) = sum_t
-// Ypart[t][l]. Its share of v^T A22 v goes to pyv[tile]. Blocks past the
-// tiles form the panel dots W(:,q).v and V(:,q).v. Every block forms the
-// reflector of column j from the norm partials (identically); block 0
-// stores d, e, tau; the grid writes v into the panel (and its mirror V2).
+// local index l = r - o, size m) with column j itself, a(l) = A[o + l, j],
+// reading only the 64 x 64 tiles on and below the diagonal (both triangles
+// of a diagonal tile are stored, so it is used whole). The reflector is
+// v = s (a - beta e0), s = 1 / (alpha - beta), so
+//   A22 v = s (A22 a - beta A22(:, 0)),
+//   v^T A22 v = s^2 (a^T A22 a - 2 beta (A22 a)(0) + beta^2 A22(0, 0)),
+// and the tiles need neither the norm nor beta: the post kernel applies s
+// and beta. Tile (I, J), I >= J, writes its row partial
+// sum_{c in J} A[r,c] a[c] to Ypart[J][r] and, below the diagonal, its
+// column partial sum_{r in I} A[r,c] a[r] to Ypart[I][c]: every (t, l) of
+// Ypart (t < T tile rows) has exactly one writer and (A22 a)(l) =
+// sum_t Ypart[t][l]. Its share of a^T A22 a goes to pyv[tile].
+//
+// Thread (wave w, lane l) keeps A[r0 + l, c0 + w + 4q], q < 16, in
+// registers. Row partials are 16 products per thread summed over the four
+// waves in LDS; column partials are sums over the 64 lanes, done for all 16
+// columns of a wave at once by a transposing butterfly (each exchange halves
+// the values a lane carries), after which lanes with l % 4 == 0 hold column
+// w + 4 q(l), q(l) = 8 b5 + 4 b4 + 2 b3 + b2 from the lane's bits. With no
+// tile in LDS eight blocks fit on a CU, so a 4096-row trailing block (2080
+// tiles) is read in one round.
+//
+// The first i blocks form the panel dots W(:,q).v and V(:,q).v, q < i; they,
+// and the blocks that write v into the panel (and its mirror V2), form the
+// reflector from the norm partials (identically in every block). Block 0
+// stores d, e, tau and sc = {s, A22(0, 0)}.
 constexpr int TS = 64;
 
-__global__ __launch_bounds__(256) void trd_symv_tiles_kernel(
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void trd_symv_kernel(
     const double* __restrict__ A, int n, int j, int i, const double* __restrict__ pnorm, int npn,
     double* __restrict__ Vp, const double* __restrict__ Wp, double* __restrict__ V2,
     double* __restrict__ Ypart, double* __restrict__ ab, double* __restrict__ pyv, int ntiles,
-    double* d, double* e, double* tau) {
-  __shared__ double tile[TS][TS + 1];   // [column][row]
+    double* d, double* e, double* tau, double* __restrict__ sc) {
   __shared__ double vr[TS], vc[TS];
   __shared__ double red[4][TS];
   __shared__ double sh4[4];
@@ -128,40 +144,97 @@ __global__ __launch_bounds__(256) void trd_symv_tiles_kernel(
   const double* a = A + (size_t)j * ld;
   const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
   const int m = n - j - 1, o = j + 1;
-  const int bid = blockIdx.x;
-  // tile coordinates and the tile's loads first: they do not depend on the
-  // reflector, so their latency overlaps the norm reduction
-  int I = 0, J = 0;
-  double tv0 = 0, tv1 = 0, tv2 = 0, tv3 = 0, tv4 = 0, tv5 = 0, tv6 = 0, tv7 = 0, tv8 = 0, tv9 = 0,
-         tv10 = 0, tv11 = 0, tv12 = 0, tv13 = 0, tv14 = 0, tv15 = 0, araw = 0;
-  if (bid < ntiles) {
-    I = (int)((sqrt(8.0 * bid + 1.0) - 1.0) * 0.5);
+  // the panel-dot blocks come first: each walks a whole column, so they
+  // start before the tiles fill the machine
+  const int bid = (int)blockIdx.x - i;   // tile index, negative for the dots
+  const int rv0 = blockIdx.x * 256 + t;
+  const bool writes_v = (int)blockIdx.x * 256 < n;
+  if (bid >= 0) {
+    int I = (int)((sqrt(8.0 * bid + 1.0) - 1.0) * 0.5);
     while ((I + 1) * (I + 2) / 2 <= bid) ++I;
     while (I * (I + 1) / 2 > bid) --I;
-    J = bid - I * (I + 1) / 2;
-    const int lr = TS * I + lane;
-    const bool rok = lr < m;
-    const double* src = A + (size_t)(o + (rok ? lr : 0)) + (size_t)o * ld;
-#define TLD(q) (rok && TS * J + wv + 4 * (q) < m ? src[(size_t)(TS * J + wv + 4 * (q)) * ld] : 0.0)
-    tv0 = TLD(0); tv1 = TLD(1); tv2 = TLD(2); tv3 = TLD(3);
-    tv4 = TLD(4); tv5 = TLD(5); tv6 = TLD(6); tv7 = TLD(7);
-    tv8 = TLD(8); tv9 = TLD(9); tv10 = TLD(10); tv11 = TLD(11);
-    tv12 = TLD(12); tv13 = TLD(13); tv14 = TLD(14); tv15 = TLD(15);
-#undef TLD
-    if (t < TS) {
-      int l = TS * I + t;
-      araw = l < m ? a[o + l] : 0.0;
-    } else if (t < 2 * TS) {
-      int l = TS * J + t - TS;
-      araw = l < m ? a[o + l] : 0.0;
+    const int J = bid - I * (I + 1) / 2;
+    const int r0 = TS * I, c0 = TS * J;
+    const bool rok = r0 + lane < m;
+    // wave-uniform column pointers with a per-lane row offset
+    const int wu = __builtin_amdgcn_readfirstlane(wv);
+    const double* base = A + (size_t)(o + r0) + (size_t)(o + c0 + wu) * ld;
+    double x[16];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      const int c = c0 + wu + 4 * q;
+      x[q] = (rok && c < m) ? base[(size_t)(4 * q) * ld + lane] : 0.0;
     }
+    if (t < TS) {
+      vr[t] = r0 + t < m ? a[o + r0 + t] : 0.0;
+    } else if (t < 2 * TS) {
+      vc[t - TS] = c0 + t - TS < m ? a[o + c0 + t - TS] : 0.0;
+    }
+    __syncthreads();
+    double s = 0.0;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) s += x[q] * vc[wv + 4 * q];
+    red[wv][lane] = s;
+    if (I > J) {
+      const double vl = vr[lane];
+#pragma unroll
+      for (int q = 0; q < 16; ++q) x[q] *= vl;
+      // 16 values -> 8 -> 4 -> 2 -> 1 across lane bits 5, 4, 3, 2
+      {
+        const bool hi = (lane & 32) != 0;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+          const double send = hi ? x[k] : x[k + 8];
+          const double keep = hi ? x[k + 8] : x[k];
+          x[k] = keep + __shfl_xor(send, 32, 64);
+        }
+      }
+      {
+        const bool hi = (lane & 16) != 0;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          const double send = hi ? x[k] : x[k + 4];
+          const double keep = hi ? x[k + 4] : x[k];
+          x[k] = keep + __shfl_xor(send, 16, 64);
+        }
+      }
+      {
+        const bool hi = (lane & 8) != 0;
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+          const double send = hi ? x[k] : x[k + 2];
+          const double keep = hi ? x[k + 2] : x[k];
+          x[k] = keep + __shfl_xor(send, 8, 64);
+        }
+      }
+      {
+        const bool hi = (lane & 4) != 0;
+        const double send = hi ? x[0] : x[1];
+        const double keep = hi ? x[1] : x[0];
+        x[0] = keep + __shfl_xor(send, 4, 64);
+      }
+      double z = x[0];
+      z += __shfl_xor(z, 2, 64);
+      z += __shfl_xor(z, 1, 64);
+      const int q = ((lane >> 5) & 1) * 8 + ((lane >> 4) & 1) * 4 + ((lane >> 3) & 1) * 2 +
+                    ((lane >> 2) & 1);
+      const int c = c0 + wv + 4 * q;
+      if ((lane & 3) == 0 && c < m) Ypart[(size_t)I * m + c] = z;
+    }
+    __syncthreads();
+    if (wv == 0) {
+      const double rowp = (red[0][lane] + red[1][lane]) + (red[2][lane] + red[3][lane]);
+      if (r0 + lane < m) Ypart[(size_t)J * m + r0 + lane] = rowp;
+      double pv = wave_sum(rowp * vr[lane]);
+      if (lane == 0) pyv[bid] = (I > J ? 2.0 : 1.0) * pv;
+    }
+    if (!writes_v) return;
   }
-  // loads that do not wait for the reflector go out before its reduction
-  // (a barrier would otherwise order them after it)
+  // the reflector: panel dots, v into the panel, d, e, tau, sc
   const double alpha = a[j + 1];
-  const int rv0 = blockIdx.x * 256 + t;
   const double av0 = (rv0 < n && rv0 > j + 1) ? a[rv0] : 0.0;
   const double ajj = (blockIdx.x == 0 && t == 0) ? a[j] : 0.0;
+  const double a00 = (blockIdx.x == 0 && t == 0) ? A[(size_t)o + (size_t)o * ld] : 0.0;
   double xn2 = sum_partials(pnorm, npn, sh4);
   double beta = alpha, tv = 0.0, scale = 1.0;
   if (xn2 != 0.0) {
@@ -173,62 +246,40 @@ __global__ __launch_bounds__(256) void trd_symv_tiles_kernel(
     d[j] = ajj;
     e[j] = beta;
     tau[j] = tv;
+    sc[0] = scale;
+    sc[1] = a00;
   }
   for (int r = rv0; r < n; r += gridDim.x * 256) {
     double v = r <= j ? 0.0 : (r == j + 1 ? 1.0 : (r == rv0 ? av0 : a[r]) * scale);
     Vp[r + (size_t)i * ld] = v;
     V2[r + (size_t)i * ld] = v;
   }
-  if (bid < ntiles) {
-    const int r0 = TS * I, c0 = TS * J;
-    tile[wv][lane] = tv0; tile[wv + 4][lane] = tv1; tile[wv + 8][lane] = tv2;
-    tile[wv + 12][lane] = tv3; tile[wv + 16][lane] = tv4; tile[wv + 20][lane] = tv5;
-    tile[wv + 24][lane] = tv6; tile[wv + 28][lane] = tv7; tile[wv + 32][lane] = tv8;
-    tile[wv + 36][lane] = tv9; tile[wv + 40][lane] = tv10; tile[wv + 44][lane] = tv11;
-    tile[wv + 48][lane] = tv12; tile[wv + 52][lane] = tv13; tile[wv + 56][lane] = tv14;
-    tile[wv + 60][lane] = tv15;
-    if (t < TS) {
-      int l = r0 + t;
-      vr[t] = l < m ? (l == 0 ? 1.0 : araw * scale) : 0.0;
-    } else if (t < 2 * TS) {
-      int l = c0 + t - TS;
-      vc[t - TS] = l < m ? (l == 0 ? 1.0 : araw * scale) : 0.0;
-    }
-    __syncthreads();
-    double s = 0.0;
-#pragma unroll 4
-    for (int cc = wv * 16; cc < wv * 16 + 16; ++cc) s += tile[cc][lane] * vc[cc];
-    red[wv][lane] = s;
-    __syncthreads();
-    double rowp = (red[0][lane] + red[1][lane]) + (red[2][lane] + red[3][lane]);
-    if (wv == 0 && r0 + lane < m) Ypart[(size_t)J * m + r0 + lane] = rowp;
-    if (I > J) {
-      double s2 = 0.0;
-#pragma unroll 4
-      for (int rr = wv * 16; rr < wv * 16 + 16; ++rr) s2 += tile[lane][rr] * vr[rr];
-      __syncthreads();   // every wave has read red
-      red[wv][lane] = s2;
-      __syncthreads();
-      if (wv == 0 && c0 + lane < m)
-        Ypart[(size_t)I * m + c0 + lane] =
-            (red[0][lane] + red[1][lane]) + (red[2][lane] + red[3][lane]);
-    }
-    if (wv == 0) {
-      double pv = wave_sum(rowp * vr[lane]);
-      if (lane == 0) pyv[bid] = (I > J ? 2.0 : 1.0) * pv;
-    }
-  } else {
-    const int q = bid - ntiles;   // q < i by the grid size
+  if (bid < 0) {
+    const int q = (int)blockIdx.x;   // q < i
     const double* wq = Wp + (size_t)q * ld + o;
     const double* vq = Vp + (size_t)q * ld + o;
-    double s0 = 0.0, s1 = 0.0;
-    for (int l = t; l < m; l += 256) {
-      double vv = l == 0 ? 1.0 : a[o + l] * scale;
+    const double* ac = a + o;
+    double s0 = 0.0, s1 = 0.0, u0 = 0.0, u1 = 0.0;
+    int l = t;
+    // two rows per thread in flight
+    for (; l + 256 < m; l += 2 * 256) {
+      const double a0 = ac[l], a1 = ac[l + 256];
+      const double w0 = wq[l], w1 = wq[l + 256];
+      const double p0 = vq[l], p1 = vq[l + 256];
+      const double v0 = l == 0 ? 1.0 : a0 * scale;
+      const double v1 = a1 * scale;
+      s0 += w0 * v0;
+      u0 += w1 * v1;
+      s1 += p0 * v0;
+      u1 += p1 * v1;
+    }
+    for (; l < m; l += 256) {
+      const double vv = l == 0 ? 1.0 : ac[l] * scale;
       s0 += wq[l] * vv;
       s1 += vq[l] * vv;
     }
-    s0 = block_sum256(s0, sh4);
-    s1 = block_sum256(s1, sh4);
+    s0 = block_sum256(s0 + u0, sh4);
+    s1 = block_sum256(s1 + u1, sh4);
     if (t == 0) {
       ab[q] = s0;
       ab[i + q] = s1;
@@ -262,16 +313,20 @@ constexpr int POST_ROWS = 16;   // rows per block of the post kernel
 // next panel column's update and its squared-tail partial sum. Thread
 // (row rr = t % 16, part = t / 16) sums its share t = part (mod 16) of the
 // symv row partials and q = part (mod 16) of the panel terms; the 16 parts
-// meet in LDS once. The scalars (y^T v, a^T b, y(j+1), the panel dot at
-// j+1) are formed the same way in every block, so all blocks agree bit for
-// bit.
+// meet in LDS once. The symv partials are of A22 a; with the reflector's
+// scale s = sc[0] and beta = e[j], y(r) = s ((A22 a)(r) - beta A(r, j+1)),
+// and v^T A22 v = s^2 (a^T A22 a - 2 beta (A22 a)(0) + beta^2 sc[1]),
+// sc[1] = A(j+1, j+1). The scalars (v^T A22 v, a^T b, y(j+1), the panel
+// dot at j+1) are formed the same way in every block, so all blocks agree
+// bit for bit.
 __global__ __launch_bounds__(256) void trd_post_kernel(
     double* __restrict__ A, int n, int j, int i, int nsym, const double* __restrict__ Vp,
     double* __restrict__ Wp, const double* __restrict__ Ypart, int T,
     const double* __restrict__ ab, const double* __restrict__ pyv,
-    const double* __restrict__ tau, int next, double* __restrict__ pnorm, int b0) {
+    const double* __restrict__ tau, const double* __restrict__ e,
+    const double* __restrict__ sc, int next, double* __restrict__ pnorm, int b0) {
   __shared__ double red[3][POST_ROWS][POST_ROWS + 1];
-  __shared__ double sc[4][4];
+  __shared__ double scr[4][4];
   const size_t ld = (size_t)n;
   const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
   const int rr = t % POST_ROWS, part = t / POST_ROWS;
@@ -317,11 +372,11 @@ __global__ __launch_bounds__(256) void trd_post_kernel(
       s1 = Vp[jn + (size_t)lane * ld] * ab[lane] + Wp[jn + (size_t)lane * ld] * ab[i + lane];
   }
   // wave 0's operands of the final step, loaded before the barrier
-  const double tj = tau[j];
+  const double tj = tau[j], bj = e[j], sj = sc[0], a00 = sc[1];
   double vr0 = 0.0, ajn0 = 0.0;
   if (wv == 0 && lane < POST_ROWS && r < n) {
     vr0 = Vp[r + (size_t)i * ld];
-    if (next && r >= jn) ajn0 = A[r + (size_t)jn * ld];
+    if (r >= jn) ajn0 = A[r + (size_t)jn * ld];
   }
   pv = wave_sum(pv);
   s1 = wave_sum(s1);
@@ -329,23 +384,26 @@ __global__ __launch_bounds__(256) void trd_post_kernel(
   red[1][part][rr] = sw;
   red[2][part][rr] = sn;
   if (lane == 0) {
-    sc[0][wv] = pv;
-    sc[1][wv] = s1;
+    scr[0][wv] = pv;
+    scr[1][wv] = s1;
   }
   __syncthreads();
   if (wv != 0) return;
-  const double ytv = (sc[0][0] + sc[0][1]) + (sc[0][2] + sc[0][3]);
-  const double y0 = sc[1][1], atb = sc[1][2], djn = sc[1][3];
+  const double ata = (scr[0][0] + scr[0][1]) + (scr[0][2] + scr[0][3]);
+  const double y0a = scr[1][1], atb = scr[1][2], djn = scr[1][3];
+  const double ytv = sj * sj * ((ata - 2.0 * bj * y0a) + bj * bj * a00);
+  const double y0 = sj * (y0a - bj * a00);
   const double alpha = -0.5 * tj * tj * (ytv - 2.0 * atb);
   double s = 0.0;
   if (lane < POST_ROWS && r < n) {
-    double yr = 0.0, swr = 0.0, snr = 0.0;
+    double ya = 0.0, swr = 0.0, snr = 0.0;
 #pragma unroll
     for (int p = 0; p < POST_ROWS; ++p) {
-      yr += red[0][p][lane];
+      ya += red[0][p][lane];
       swr += red[1][p][lane];
       snr += red[2][p][lane];
     }
+    const double yr = sj * (ya - bj * ajn0);
     const double vr = vr0;
     double wr = 0.0;
     if (r > j && tj != 0.0) wr = tj * (yr - swr) + alpha * vr;
@@ -1095,7 +1153,8 @@ static int eig_workspace(itg_ctx* ctx, size_t bytes, void** out) {
 // Tridiagonalise A (n x n FP64, full symmetric storage) in place: d, e out;
 // reflectors left in A's strictly lower part (v_j(j+1) = 1 explicit), tau out.
 static int tridiagonalize(itg_ctx* ctx, int n, double* A, double* d, double* e, double* tau,
-                          double* VW, double* Ypart, double* ab, double* pnorm, double* pyv) {
+                          double* VW, double* Ypart, double* ab, double* pnorm, double* pyv,
+                          double* sc) {
   // VW = [V | W | V], n x NB each: the trailing update of a full panel is one
   // K = 2 NB product A22 -= [V W] [W V]^T
   double* Vp = VW;
@@ -1131,10 +1190,10 @@ static int tridiagonalize(itg_ctx* ctx, int n, double* A, double* d, double* e, 
       int T = (m + TS - 1) / TS;
       int ntiles = T * (T + 1) / 2;
       itg_tev tk = itg_tevents(ctx, ITG_K_TRD_SYMV);
-      hipExtLaunchKernelGGL(trd_symv_tiles_kernel, dim3(ntiles + i), dim3(256), 0, ctx->stream,
+      hipExtLaunchKernelGGL(trd_symv_kernel, dim3(ntiles + i), dim3(256), 0, ctx->stream,
                             tk.a, tk.b, 0, (const double*)A, n, j, i,
                             (const double*)(pnorm + pn_off), pn_cnt, Vp, (const double*)Wp, V2,
-                            Ypart, ab, pyv, ntiles, d, e, tau);
+                            Ypart, ab, pyv, ntiles, d, e, tau, sc);
       ITG_LAUNCH_CHECK(ctx);
       // lower triangle of A22 read once, the panel pairs over m rows
       itg_tcommit(ctx, tk, ITG_K_TRD_SYMV, 2.0 * m * (m + 2.0 * i),
@@ -1145,8 +1204,8 @@ static int tridiagonalize(itg_ctx* ctx, int n, double* A, double* d, double* e, 
       itg_tev tp = itg_tevents(ctx, ITG_K_TRD_POST);
       hipExtLaunchKernelGGL(trd_post_kernel, dim3(nblk), dim3(256), 0, ctx->stream, tp.a, tp.b, 0,
                             A, n, j, i, ntiles, (const double*)Vp, Wp, (const double*)Ypart, T,
-                            (const double*)ab, (const double*)pyv, (const double*)tau, next,
-                            pnorm, b0);
+                            (const double*)ab, (const double*)pyv, (const double*)tau,
+                            (const double*)e, (const double*)sc, next, pnorm, b0);
       ITG_LAUNCH_CHECK(ctx);
       // the next column's reflector norm: partials of these blocks only
       pn_off = b0;
@@ -1377,6 +1436,7 @@ extern "C" int itg_eigh_trunc(itg_ctx* ctx, int dtype, int64_t n64, void* Ain, d
   double* ab = cv.take<double>(2 * NB);
   double* pnorm = cv.take<double>((size_t)n / POST_ROWS + 1);
   double* pyv = cv.take<double>(Tmax * (Tmax + 1) / 2 + 64);
+  double* trd_sc = cv.take<double>(2);
   double* d = cv.take<double>(n);
   double* e = cv.take<double>(n);
   double* tau = cv.take<double>(n);
@@ -1406,7 +1466,7 @@ extern "C" int itg_eigh_trunc(itg_ctx* ctx, int dtype, int64_t n64, void* Ain, d
   // 1. tridiagonalise
   if (cplx) rc = itg_zhetrd(ctx, n, Ain, d, e, ztau, zscr);
   else if (two) rc = eig2::reduce(ctx, n, A, d, e, eb);
-  else rc = tridiagonalize(ctx, n, A, d, e, tau, VW, Ypart, ab, pnorm, pyv);
+  else rc = tridiagonalize(ctx, n, A, d, e, tau, VW, Ypart, ab, pnorm, pyv, trd_sc);
   if (rc) return rc;
   // 2. divide and conquer
   double *Q, *D;
