@@ -663,7 +663,7 @@ def main():
                                   "every launch of the factorisation of bond %d in the first "
                                   "timed sweep (n=%d)" % (args.N // 2, 2 * args.chi))
         roof_symv["kernel"] = sname
-        tpath = os.path.join(ROOT, "profiles", "trd_symv_tiles_kernel_traffic_n4096.json")
+        tpath = os.path.join(ROOT, "profiles", "trd_symv_kernel_traffic_n4096.json")
         if os.path.exists(tpath):
             tr = json.load(open(tpath))
             roof_symv["traffic"] = tr["bytes_per_launch"]
