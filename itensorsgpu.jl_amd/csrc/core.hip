@@ -48,6 +48,8 @@ extern "C" int itg_init(int device, itg_ctx** out) {
     return ITG_EHIP;
   }
   ctx->stream = ctx->own_stream;
+  const char* lib = getenv("ITG_GEMM_LIB");
+  ctx->libgemm = (lib && lib[0] == '0') ? 0 : 1;
   if (hipMalloc(&ctx->red, kRedSlots * sizeof(double)) != hipSuccess ||
       hipHostMalloc(&ctx->red_host, kRedSlots * sizeof(double), hipHostMallocDefault) != hipSuccess) {
     hipStreamDestroy(ctx->own_stream);
@@ -72,6 +74,7 @@ extern "C" int itg_destroy(itg_ctx* ctx) {
   itg_plan_cache_clear(ctx);
   itg_graph_cache_clear(ctx);
   itg_timing_free(ctx);
+  itg_libgemm_free(ctx);
   if (ctx->ws) hipFree(ctx->ws);
   if (ctx->eig_ws) hipFree(ctx->eig_ws);
   if (ctx->mlp_ws) hipFree(ctx->mlp_ws);

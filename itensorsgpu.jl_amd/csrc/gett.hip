@@ -671,6 +671,44 @@ static int launch_gett(itg_ctx* ctx, int dtype, int mode, GettArgs& g, bool ak, 
   return launch_gett_t<float, ITG_MODE_NATIVE>(ctx, g, ak, bk);
 }
 
+// A contraction whose M, N and K modes each merge into one stride is a plain
+// product C(m, n) = alpha sum_k A(m, k) B(k, n) + beta C(m, n), elements at
+// m sAm + k sAk, k sBk + n sBn, m sCm + n sCn. It runs as a column-major
+// GEMM (the vendor library's) when each matrix has a unit stride on one
+// side: X(r, c) at r sr + c sc is op 'N' with ld = sc when sr = 1, op 'T'
+// with ld = sr when sc = 1. If C's unit stride is along n, C^T = B^T A^T is
+// computed instead. Returns 1 for a layout that is not of this kind.
+static int contract_on_lib(itg_ctx* ctx, const GettArgs& g) {
+  auto as_col = [](int64_t rows, int64_t cols, int64_t sr, int64_t sc, char& op,
+                   int64_t& ld) -> bool {
+    if ((rows == 1 || sr == 1) && (cols == 1 || sc >= rows)) {
+      op = 'N';
+      ld = cols == 1 ? rows : sc;
+      return true;
+    }
+    if ((cols == 1 || sc == 1) && (rows == 1 || sr >= cols)) {
+      op = 'T';
+      ld = rows == 1 ? cols : sr;
+      return true;
+    }
+    return false;
+  };
+  char oa, ob, oc;
+  int64_t la, lb, lc;
+  if (!as_col(g.M, g.N, g.sCm, g.sCn, oc, lc)) return 1;
+  if (oc == 'N') {
+    if (as_col(g.M, g.K, g.sAm, g.sAk, oa, la) && as_col(g.K, g.N, g.sBk, g.sBn, ob, lb))
+      return itg_libgemm(ctx, oa, ob, g.M, g.N, g.K, g.alpha, (const double*)g.A, la,
+                         (const double*)g.B, lb, g.beta, (double*)g.C, lc);
+  } else {
+    // C^T (n x m, column-major with ld lc) = B^T A^T
+    if (as_col(g.N, g.K, g.sBn, g.sBk, ob, lb) && as_col(g.K, g.M, g.sAk, g.sAm, oa, la))
+      return itg_libgemm(ctx, ob, oa, g.N, g.M, g.K, g.alpha, (const double*)g.B, lb,
+                         (const double*)g.A, la, g.beta, (double*)g.C, lc);
+  }
+  return 1;
+}
+
 static int scale_output(itg_ctx* ctx, int dtype, int rC, const int64_t* dC, double beta, void* C) {
   int64_t n = 1;
   for (int i = 0; i < rC; ++i) n *= dC[i];
@@ -754,6 +792,11 @@ extern "C" int itg_contract(itg_ctx* ctx, int dtype, int mode, int rA, const int
   g.tBn = tp(p->oBn); g.tCm = tp(p->oCm); g.tCn = tp(p->oCn);
   g.alpha = alpha;
   g.beta = beta;
+  if (!g.tAm && !g.tAk && !g.tBk && !g.tBn && !g.tCm && !g.tCn &&
+      itg_libgemm_ok(ctx, dtype, mode, g.M, g.N, g.K)) {
+    int rc = contract_on_lib(ctx, g);
+    if (rc != 1) return rc;
+  }
   return launch_gett(ctx, dtype, mode, g, p->akfast, p->bkfast);
 }
 
@@ -770,6 +813,10 @@ int itg_gemm_internal(itg_ctx* ctx, int dtype, int mode, char ta, char tb, int64
     return itg_fail(ctx, ITG_EUNSUPPORTED, "k == 0 with ldc != m");
   }
   bool tA = (ta == 'T' || ta == 't'), tB = (tb == 'T' || tb == 't');
+  if (!lower_band && lda >= (tA ? k : m) && ldb >= (tB ? n : k) && ldc >= m &&
+      itg_libgemm_ok(ctx, dtype, mode, m, n, k))
+    return itg_libgemm(ctx, ta, tb, m, n, k, alpha, (const double*)A, lda, (const double*)B, ldb,
+                       beta, (double*)C, ldc);
   GettArgs g;
   memset(&g, 0, sizeof g);
   // C(m,n) at m + n*ldc: its unit stride is m, so compute C^T = B^T A^T
@@ -824,8 +871,13 @@ extern "C" int itg_gemm(itg_ctx* ctx, int dtype, int mode, char ta, char tb, int
     // C = alpha A A^T (or A^T A), the density matrix of a bond: the lower
     // triangle (and one tile band) on MFMA, the rest mirrored, so C is
     // exactly symmetric
-    int rc = itg_gemm_internal(ctx, dtype, mode, ta, tb, m, n, k, alpha, A, lda, B, ldb, beta, C,
-                               ldc, true);
+    int rc;
+    if (lda >= (nt ? m : k) && itg_libgemm_ok(ctx, dtype, mode, m, n, k))
+      rc = itg_libsyrk_lower(ctx, nt ? 'N' : 'T', m, k, alpha, (const double*)A, lda, (double*)C,
+                             ldc);
+    else
+      rc = itg_gemm_internal(ctx, dtype, mode, ta, tb, m, n, k, alpha, A, lda, B, ldb, beta, C,
+                             ldc, true);
     if (rc) return rc;
     dim3 grid((unsigned)((m + 31) / 32), (unsigned)((m + 31) / 32));
     if (dtype == ITG_F64)

@@ -65,6 +65,10 @@ struct itg_ctx {
   // grow-only device workspace of the complex contraction's expanded operand
   void* cws = nullptr;
   size_t cws_bytes = 0;
+  // vendor BLAS handle for plain FP64 GEMMs (created on first use) and
+  // whether they go there (ITG_GEMM_LIB=0 at init keeps them on GETT)
+  void* blas = nullptr;
+  int libgemm = 1;
 };
 
 constexpr int kRedSlots = 16384;
@@ -124,6 +128,18 @@ int itg_gemm_internal(itg_ctx* ctx, int dtype, int mode, char ta, char tb,
                       const void* A, int64_t lda, const void* B, int64_t ldb,
                       double beta, void* C, int64_t ldc, bool lower_band = false);
 void itg_plan_cache_clear(itg_ctx* ctx);
+// Plain FP64 GEMMs on the vendor library (libgemm.hip). itg_libgemm_ok says
+// whether a native FP64 product of this size may go there (not while the
+// stream is being captured, not inside a no-alloc call, dimensions in 32
+// bits); itg_libgemm is column-major C = alpha op(A) op(B) + beta C and
+// itg_libsyrk the lower triangle of C = alpha op(A) op(A)^T.
+bool itg_libgemm_ok(itg_ctx* ctx, int dtype, int mode, int64_t m, int64_t n, int64_t k);
+int itg_libgemm(itg_ctx* ctx, char ta, char tb, int64_t m, int64_t n, int64_t k, double alpha,
+                const double* A, int64_t lda, const double* B, int64_t ldb, double beta,
+                double* C, int64_t ldc);
+int itg_libsyrk_lower(itg_ctx* ctx, char trans, int64_t n, int64_t k, double alpha,
+                      const double* A, int64_t lda, double* C, int64_t ldc);
+void itg_libgemm_free(itg_ctx* ctx);
 // Event bracket of one launch for itg_timing: t = itg_tstart(ctx, kind)
 // before the launch, itg_tstop(ctx, t, kind, flops, bytes) after it (no-ops
 // unless the kind is being timed).

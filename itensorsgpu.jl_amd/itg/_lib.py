@@ -22,7 +22,7 @@ MODES = {"fp64": MODE_NATIVE, "native": MODE_NATIVE, "fp32": MODE_F32, "bf16x3":
 # kernel classes of itg_timing
 K_GEMM_SPLIT, K_GETT, K_WPASS, K_TRD_SYMV, K_TRD_POST = 0, 1, 2, 3, 4
 KERNEL_NAMES = {K_GEMM_SPLIT: "gemm_split_kernel (split-BF16 GEMM)",
-                K_GETT: "gett_kernel (contraction GEMM)",
+                K_GETT: "contraction GEMM (GETT kernel or rocBLAS dgemm)",
                 K_WPASS: "heff_wpass_kernel (W1.W2 pass)",
                 K_TRD_SYMV: "trd_symv_kernel (tridiagonalisation symv)",
                 K_TRD_POST: "trd_post_kernel (tridiagonalisation update)"}

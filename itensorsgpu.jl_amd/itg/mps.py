@@ -334,16 +334,19 @@ def _bra(A):
 
 def update_left(L, Ld, A, Ad, W, Wd, mode="native"):
     """L_new(r', wr, r) = L(l', wl, l) A(l, s, r) W(wl, wr, s', s) conj(A(l', s', r'))."""
+    # T2's order (l', s', wr, r) makes the last contraction, like the first,
+    # a plain GEMM: (l' s') contiguous, (wr r) one stride
     T1, d1 = ops.contract(L, Ld, [1, 2, 3], A, Ad, [3, 4, 5], [1, 2, 4, 5], mode=mode)
-    T2, d2 = ops.contract(T1, d1, [1, 2, 4, 5], W, Wd, [2, 6, 7, 4], [1, 7, 5, 6])
-    return ops.contract(T2, d2, [1, 7, 5, 6], _bra(A), Ad, [1, 7, 8], [8, 6, 5], mode=mode)
+    T2, d2 = ops.contract(T1, d1, [1, 2, 4, 5], W, Wd, [2, 6, 7, 4], [1, 7, 6, 5])
+    return ops.contract(T2, d2, [1, 7, 6, 5], _bra(A), Ad, [1, 7, 8], [8, 6, 5], mode=mode)
 
 
 def update_right(R, Rd, A, Ad, W, Wd, mode="native"):
     """R_new(l', wl, l) = R(r', wr, r) A(l, s, r) W(wl, wr, s', s) conj(A(l', s', r'))."""
+    # T2's order (s', r', wl, l) makes the last contraction a plain GEMM
     T1, d1 = ops.contract(R, Rd, [1, 2, 3], A, Ad, [4, 5, 3], [1, 2, 4, 5], mode=mode)
-    T2, d2 = ops.contract(T1, d1, [1, 2, 4, 5], W, Wd, [6, 2, 7, 5], [1, 7, 4, 6])
-    return ops.contract(T2, d2, [1, 7, 4, 6], _bra(A), Ad, [8, 7, 1], [8, 6, 4], mode=mode)
+    T2, d2 = ops.contract(T1, d1, [1, 2, 4, 5], W, Wd, [6, 2, 7, 5], [7, 1, 6, 4])
+    return ops.contract(T2, d2, [7, 1, 6, 4], _bra(A), Ad, [8, 7, 1], [8, 6, 4], mode=mode)
 
 
 def heff_apply_generic(L, Ld, W1, W1d, W2, W2d, R, Rd, x, xdims):

@@ -239,21 +239,41 @@ def test_guard_reverts_a_useless_network(itg):
 
 
 def test_guard_keeps_residual_decreasing_with_garbage_weights(itg):
-    """Large random weights: accepted learned steps must each lower the
-    residual (the guard), so the final residual is below the start's."""
+    """Large random weights: every learned step the guard accepts lowers the
+    residual. The calls show the steps: the learned correction is asked for
+    at each iteration's residual q; a fallback call at the same q means the
+    learned step was reverted, a learned call at the next iteration means it
+    was kept, and then the new residual must be smaller. (The fallback step
+    itself is not guarded, so the final residual is not bounded by the
+    first one.)"""
     from itg import mps, ops
     from itg.precond import MLPPreconditioner
     net = MLPPreconditioner(2, hidden=32, seed=9)
     net.set_params(np.random.default_rng(9).standard_normal(net.nparam).astype(np.float32) * 30)
     (mv, phi, dg, pdims), = _chain_operators(itg, 12, 16, 6, {6})
-    x = ops.copy(phi)
-    ops.scal(x, 1.0 / ops.nrm2(x))
-    r0 = ops.copy(mv(x))
-    ops.axpby(-ops.dot(x, r0), x, 1.0, r0)
+    calls = []
+    diag = mps.diag_preconditioner(dg)
+
+    def learned(q, lam):
+        calls.append(("learned", ops.nrm2(q)))
+        return net.correction(q, lam, dg, pdims)
+
+    def fallback(q, lam):
+        calls.append(("fallback", ops.nrm2(q)))
+        return diag(q, lam)
+
     st = {}
-    lam, _ = mps.davidson(mv, phi, maxiter=3, stats=st, fallback=mps.diag_preconditioner(dg),
-                          precond=lambda q, lam: net.correction(q, lam, dg, pdims))
-    assert st["residual"] < ops.nrm2(r0)
+    mps.davidson(mv, phi, maxiter=6, stats=st, fallback=fallback, precond=learned)
+    kept = 0
+    for (k0, r0), (k1, r1) in zip(calls, calls[1:]):
+        if k0 == "learned" and k1 == "learned":
+            assert r1 < r0
+            kept += 1
+    if calls and calls[-1][0] == "learned":
+        assert st["residual"] < calls[-1][1]
+    # each iteration asks the learned step once; reverted ones also the fallback
+    assert st["reverted"] == sum(1 for k, _ in calls if k == "fallback")
+    assert st["reverted"] + kept + (1 if calls and calls[-1][0] == "learned" else 0) >= 1
 
 
 def test_learned_preconditioner_gain(itg, capsys):
