@@ -142,55 +142,84 @@ ITG_HD static inline double dc_zhat(int i, int k, const double* dl, const int* o
 // (|z| = 1), col (column of Q holding each entry's vector); rho > 0.
 // Outputs: non-deflated (dl, w, gcol)[0..k); deflated (dval, dcol)[0..nd);
 // rotations (rcol pairs, rcs = (c, s)) to apply to Q's columns in order:
-// q_p <- c q_p + s q_n, q_n <- c q_n - s q_p. Returns k.
-ITG_HD static inline int dc_deflate(int s, double* d, double* z, const int* col, double rho,
-                                    double* dl, double* w, int* gcol, double* dval, int* dcol,
-                                    int* nd_out, int* rcol, double* rcs, int* nrot_out) {
+// q_p <- c q_p + s q_n, q_n <- c q_n - s q_p. Returns k. d and z are read
+// only.
+//
+// The rules are dlaed2's: an entry with rho |z| <= tol deflates; of two
+// consecutive kept entries p, j with |t c s| <= tol (t = d_j - d_p,
+// c = z_j / tau, s = -z_p / tau, tau^2 = z_p^2 + z_j^2) the rotation moves
+// p's weight onto j, p deflates at d_p c^2 + d_j s^2 and j goes on at
+// d_p s^2 + d_j c^2 = (d_p z_p^2 + d_j z_j^2) / tau^2 with z_j = tau. Along a
+// run of such rotations the entry carried is thus the z^2-weighted mean of
+// the run's d with weight sqrt(sum z^2): the pass keeps N = sum d z^2 and
+// Z2 = sum z^2 and makes the test in the squared form
+// (d_j Z2 - N)^2 z_j^2 <= tol^2 Z2 (Z2 + z_j^2)^2 (|t c s| <= tol squared
+// and multiplied through by Z2 (Z2 + z_j^2)^2 > 0), so the step from one
+// entry to the next is a few multiply-adds; square roots and divisions are
+// only taken for the values written out. A run of one entry keeps its own d
+// and signed z.
+ITG_HD static inline int dc_deflate(int s, const double* d, const double* z, const int* col,
+                                    double rho, double* dl, double* w, int* gcol, double* dval,
+                                    int* dcol, int* nd_out, int* rcol, double* rcs,
+                                    int* nrot_out) {
   double dmax = 0.0, zmax = 0.0;
   for (int j = 0; j < s; ++j) {
     dmax = fmax(dmax, fabs(d[j]));
     zmax = fmax(zmax, fabs(z[j]));
   }
-  double tol = 8.0 * ITG_DC_EPS * fmax(dmax, zmax);
-  int k = 0, nd = 0, nrot = 0, pj = -1;
+  const double tol = 8.0 * ITG_DC_EPS * fmax(dmax, zmax);
+  const double tol2 = tol * tol;
+  int k = 0, nd = 0, nrot = 0;
   if (rho * zmax <= tol) {
     for (int j = 0; j < s; ++j) { dval[nd] = d[j]; dcol[nd] = col[j]; ++nd; }
     *nd_out = nd;
     *nrot_out = 0;
     return 0;
   }
+  // the carried entry: column pc (-1: none yet), run sums N and Z2, and for
+  // a run of one entry its own d0, z0
+  int pc = -1;
+  bool single = true;
+  double N = 0.0, Z2 = 0.0, d0 = 0.0, z0 = 0.0;
+  double dn = s > 0 ? d[0] : 0.0, zn = s > 0 ? z[0] : 0.0;
   for (int j = 0; j < s; ++j) {
-    if (rho * fabs(z[j]) <= tol) {
-      dval[nd] = d[j]; dcol[nd] = col[j]; ++nd;
+    const double dj = dn, zj = zn;
+    if (j + 1 < s) {   // the next entry's loads go out before this one's work
+      dn = d[j + 1];
+      zn = z[j + 1];
+    }
+    if (rho * fabs(zj) <= tol) {
+      dval[nd] = dj; dcol[nd] = col[j]; ++nd;
       continue;
     }
-    if (pj < 0) { pj = j; continue; }
-    // |z| <= 1 here and kept entries have rho |z| > tol, so the plain norm
-    // neither overflows nor underflows
-    double S = z[pj], Cc = z[j];
-    double tau = sqrt(Cc * Cc + S * S);
-    double t = d[j] - d[pj];
-    double it = 1.0 / tau;
-    double c = Cc * it, sn = -S * it;
-    if (fabs(t * c * sn) <= tol) {
-      z[j] = tau;
-      z[pj] = 0.0;
-      rcol[2 * nrot] = col[pj];
+    const double zj2 = zj * zj;
+    if (pc < 0) {
+      pc = col[j]; single = true; d0 = dj; z0 = zj; N = dj * zj2; Z2 = zj2;
+      continue;
+    }
+    const double T2 = Z2 + zj2;
+    const double x = dj * Z2 - N;
+    if (x * x * zj2 <= tol2 * Z2 * T2 * T2) {
+      // |z| <= 1 here and kept entries have rho |z| > tol, so the plain norms
+      // neither overflow nor underflow
+      const double tau = sqrt(T2), it = 1.0 / tau;
+      const double c = zj * it, sn = -(single ? z0 : sqrt(Z2)) * it;
+      const double dp = single ? d0 : N / Z2;
+      rcol[2 * nrot] = pc;
       rcol[2 * nrot + 1] = col[j];
       rcs[2 * nrot] = c;
       rcs[2 * nrot + 1] = sn;
       ++nrot;
-      double tt = d[pj] * c * c + d[j] * sn * sn;
-      d[j] = d[pj] * sn * sn + d[j] * c * c;
-      d[pj] = tt;
-      dval[nd] = d[pj]; dcol[nd] = col[pj]; ++nd;
-      pj = j;
+      dval[nd] = dp * c * c + dj * sn * sn; dcol[nd] = pc; ++nd;
+      pc = col[j]; single = false; N += dj * zj2; Z2 = T2;
     } else {
-      dl[k] = d[pj]; w[k] = z[pj]; gcol[k] = col[pj]; ++k;
-      pj = j;
+      dl[k] = single ? d0 : N / Z2; w[k] = single ? z0 : sqrt(Z2); gcol[k] = pc; ++k;
+      pc = col[j]; single = true; d0 = dj; z0 = zj; N = dj * zj2; Z2 = zj2;
     }
   }
-  if (pj >= 0) { dl[k] = d[pj]; w[k] = z[pj]; gcol[k] = col[pj]; ++k; }
+  if (pc >= 0) {
+    dl[k] = single ? d0 : N / Z2; w[k] = single ? z0 : sqrt(Z2); gcol[k] = pc; ++k;
+  }
   *nd_out = nd;
   *nrot_out = nrot;
   return k;

@@ -82,6 +82,25 @@ def test_graded_density_matrix_spectrum(dc):
     assert np.abs(Q.T @ Q - np.eye(n)).max() < 1e-12
 
 
+def test_rank_deficient_density_matrix(dc):
+    """X Xᵀ with X of half rank: half the spectrum is a cluster at zero whose
+    neighbours rotate into each other along long runs of the deflation
+    pass (the shape of a two-site DMRG density matrix whose bond has grown
+    past its rank)."""
+    rng = np.random.default_rng(11)
+    n, r = 512, 256
+    X = rng.standard_normal((n, r)) / np.sqrt(r)
+    from scipy.linalg import hessenberg
+    H = hessenberg(X @ X.T)
+    d, e = np.diag(H).copy(), np.diag(H, -1).copy()
+    w, Q = run(dc, d, e)
+    wr = eigh_tridiagonal(d, e, eigvals_only=True)
+    assert np.abs(np.sort(w) - wr).max() <= 1e-13 * wr.max()
+    assert np.abs(Q.T @ Q - np.eye(n)).max() < 1e-12
+    T = np.diag(d) + np.diag(e, 1) + np.diag(e, -1)
+    assert np.abs(T @ Q - Q * w).max() <= 1e-12 * wr.max()
+
+
 def test_wilkinson(dc):
     m = 20
     d = np.abs(np.arange(-m, m + 1)).astype(float)
