@@ -419,6 +419,26 @@ __global__ void small_out_finish(GettArgs g, const double* part, int nblk) {
 
 // ======================================================================
 // host side: plans
+// A contraction with one small operand: the big operand X carries the free
+// modes F (at most 3) and the summed modes K, the small one Y the free modes
+// G and K, with K and G small (K <= 64, G <= 32, K G <= 1024). It streams X
+// once through sic_kernel, which forms every offset from the modes' extents
+// and strides: no offset tables (which for a non-mergeable F would have F
+// entries, built on the host for every new shape).
+constexpr int SIC_KMAX = 64, SIC_GMAX = 32;
+constexpr int64_t SIC_FMIN = 4096;   // smaller F: the GEMM kernel is fine
+struct SicArgs {
+  const void* X;
+  const void* Y;
+  void* C;
+  int64_t F;
+  int nf, ng, nk, G, K;
+  int64_t fd[3], fx[3], fc[3];   // F: extent, stride in X, stride in C
+  int64_t gd[4], gy[4], gc[4];   // G: extent, stride in Y, stride in C
+  int64_t kd[4], kx[4], ky[4];   // K: extent, stride in X, stride in Y
+  double alpha, beta;
+};
+
 struct itg_plan {
   int64_t M = 1, N = 1, K = 1;
   bool swap = false;
@@ -427,7 +447,91 @@ struct itg_plan {
   int64_t oAm = -1, oAk = -1, oBk = -1, oBn = -1, oCm = -1, oCn = -1;  // offsets into dtab
   bool akfast = false, bkfast = false;
   size_t bytes = 0;
+  // small-operand form (no tables): X is the A-role operand if sic_xa
+  bool sic = false, sic_xa = true;
+  SicArgs sa = {};
 };
+
+template <typename T, int GM>
+__global__ __launch_bounds__(256) void sic_kernel(SicArgs a) {
+  __shared__ T ys[SIC_KMAX * SIC_GMAX];   // Y(k, g) at k G + g
+  __shared__ int64_t kox[SIC_KMAX];       // offset of k in X
+  __shared__ int64_t goc[SIC_GMAX];       // offset of g in C
+  const int t = threadIdx.x;
+  const T* __restrict__ Y = (const T*)a.Y;
+  for (int e = t; e < a.K * a.G; e += 256) {
+    int64_t rk = e / a.G, rg = e % a.G, o = 0;
+    for (int q = 0; q < a.nk; ++q) { o += (rk % a.kd[q]) * a.ky[q]; rk /= a.kd[q]; }
+    for (int q = 0; q < a.ng; ++q) { o += (rg % a.gd[q]) * a.gy[q]; rg /= a.gd[q]; }
+    ys[e] = Y[o];
+  }
+  for (int k = t; k < a.K; k += 256) {
+    int64_t r = k, o = 0;
+    for (int q = 0; q < a.nk; ++q) { o += (r % a.kd[q]) * a.kx[q]; r /= a.kd[q]; }
+    kox[k] = o;
+  }
+  for (int g = t; g < a.G; g += 256) {
+    int64_t r = g, o = 0;
+    for (int q = 0; q < a.ng; ++q) { o += (r % a.gd[q]) * a.gc[q]; r /= a.gd[q]; }
+    goc[g] = o;
+  }
+  __syncthreads();
+  const int64_t f = (int64_t)blockIdx.x * 256 + t;
+  if (f >= a.F) return;
+  int64_t ox = 0, oc = 0, r = f;
+  for (int q = 0; q < a.nf; ++q) {
+    const int64_t i = r % a.fd[q];
+    r /= a.fd[q];
+    ox += i * a.fx[q];
+    oc += i * a.fc[q];
+  }
+  const T* __restrict__ X = (const T*)a.X + ox;
+  T acc[GM];
+#pragma unroll
+  for (int g = 0; g < GM; ++g) acc[g] = (T)0;
+#pragma unroll 4
+  for (int k = 0; k < a.K; ++k) {
+    const T xv = X[kox[k]];
+#pragma unroll
+    for (int g = 0; g < GM; ++g)
+      if (g < a.G) acc[g] += xv * ys[k * a.G + g];
+  }
+  T* __restrict__ C = (T*)a.C + oc;
+#pragma unroll
+  for (int g = 0; g < GM; ++g) {
+    if (g < a.G) {
+      T v = (T)a.alpha * acc[g];
+      if (a.beta != 0.0) v += (T)a.beta * C[goc[g]];
+      C[goc[g]] = v;
+    }
+  }
+}
+
+static int launch_sic(itg_ctx* ctx, int dtype, const itg_plan* p, const void* A, const void* B,
+                      void* C, double alpha, double beta) {
+  SicArgs a = p->sa;
+  a.X = p->sic_xa ? A : B;
+  a.Y = p->sic_xa ? B : A;
+  a.C = C;
+  a.alpha = alpha;
+  a.beta = beta;
+  const int64_t nblk = (a.F + 255) / 256;
+  if (nblk > 0x7fffffff) return itg_fail(ctx, ITG_EUNSUPPORTED, "contraction too large");
+  const int tk = itg_tstart(ctx, ITG_K_GETT);
+  const bool small = a.G <= 16;
+  if (dtype == ITG_F64) {
+    if (small) sic_kernel<double, 16><<<(unsigned)nblk, 256, 0, ctx->stream>>>(a);
+    else sic_kernel<double, SIC_GMAX><<<(unsigned)nblk, 256, 0, ctx->stream>>>(a);
+  } else {
+    if (small) sic_kernel<float, 16><<<(unsigned)nblk, 256, 0, ctx->stream>>>(a);
+    else sic_kernel<float, SIC_GMAX><<<(unsigned)nblk, 256, 0, ctx->stream>>>(a);
+  }
+  ITG_LAUNCH_CHECK(ctx);
+  const double es = dtype == ITG_F64 ? 8.0 : 4.0;
+  itg_tstop(ctx, tk, ITG_K_GETT, 2.0 * a.F * a.G * a.K,
+            es * (a.F * a.K + a.F * a.G * (beta != 0.0 ? 2 : 1)));
+  return ITG_OK;
+}
 
 void itg_plan_cache_clear(itg_ctx* ctx) {
   for (auto& kv : ctx->plans) {
@@ -541,6 +645,42 @@ static int build_plan(itg_ctx* ctx, const std::string& key, int rA, const int64_
   for (auto& x : cm) p->M *= x.d;
   for (auto& x : cn) p->N *= x.d;
   for (auto& x : ck) p->K *= x.d;
+  // the small-operand form: the big free group of at most 3 modes, the
+  // other free group and K small
+  if (ck.size() <= 4 && p->K <= SIC_KMAX) {
+    const bool xa = cm.size() <= 3 && cn.size() <= 4 && p->N <= SIC_GMAX &&
+                    p->N * p->K <= SIC_KMAX * SIC_GMAX && p->M >= SIC_FMIN;
+    const bool xb = !xa && cn.size() <= 3 && cm.size() <= 4 && p->M <= SIC_GMAX &&
+                    p->M * p->K <= SIC_KMAX * SIC_GMAX && p->N >= SIC_FMIN;
+    if (xa || xb) {
+      // Mode strides: M {A, C}, N {B, C}, K {A, B}
+      const std::vector<Mode>& F = xa ? cm : cn;
+      const std::vector<Mode>& G = xa ? cn : cm;
+      SicArgs& a = p->sa;
+      a.F = xa ? p->M : p->N;
+      a.G = (int)(xa ? p->N : p->M);
+      a.K = (int)p->K;
+      a.nf = (int)F.size();
+      a.ng = (int)G.size();
+      a.nk = (int)ck.size();
+      for (int q = 0; q < a.nf; ++q) { a.fd[q] = F[q].d; a.fx[q] = F[q].s1; a.fc[q] = F[q].s2; }
+      for (int q = 0; q < a.ng; ++q) { a.gd[q] = G[q].d; a.gy[q] = G[q].s1; a.gc[q] = G[q].s2; }
+      for (int q = 0; q < a.nk; ++q) {
+        a.kd[q] = ck[q].d;
+        a.kx[q] = xa ? ck[q].s1 : ck[q].s2;
+        a.ky[q] = xa ? ck[q].s2 : ck[q].s1;
+      }
+      p->sic = true;
+      p->sic_xa = xa;
+      if (ctx->plans.size() > 4096) {
+        hipStreamSynchronize(ctx->stream);
+        itg_plan_cache_clear(ctx);
+      }
+      ctx->plans[key] = p;
+      *out = p;
+      return ITG_OK;
+    }
+  }
   std::vector<int64_t> tAm, tAk, tBk, tBn, tCm, tCn;
   p->sAm = group_offsets(cm, 1, tAm);
   p->sCm = group_offsets(cm, 2, tCm);
@@ -780,6 +920,7 @@ extern "C" int itg_contract(itg_ctx* ctx, int dtype, int mode, int rA, const int
     int rc = build_plan(ctx, key, rA, dA, lA, rB, dB, lB, rC, dC, lC, &p);
     if (rc) return rc;
   }
+  if (p->sic) return launch_sic(ctx, dtype, p, p->swap ? B : A, p->swap ? A : B, C, alpha, beta);
   GettArgs g;
   memset(&g, 0, sizeof g);
   g.A = p->swap ? B : A;

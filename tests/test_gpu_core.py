@@ -241,6 +241,28 @@ def test_contract_alpha_beta_and_split_k(itg):
     check_contract(itg, x, list("abc"), y, list("cab"))
 
 
+def test_contract_small_operand(itg):
+    """Contractions with one small operand (the MPO tensor of an environment
+    update): the big free modes do not merge, so they run as a stream over
+    the big operand. Both operand roles, odd unequal bond dimensions, a
+    non-mergeable summed group, and beta != 0."""
+    rng = np.random.default_rng(22)
+    w, d = 5, 2
+    for chip, chi in ((97, 131), (131, 97), (64, 70)):
+        T1 = rng.standard_normal((chip, w, d, chi))          # (a', w, s, b)
+        W = rng.standard_normal((w, w, d, d))                # (w, w', s', s)
+        # the left update's order (a', s', w', b) and the right one's (s', a', w', b)
+        check_contract(itg, T1, list("pwsb"), W, list("wvts"), lc=list("ptvb"))
+        check_contract(itg, T1, list("pwsb"), W, list("wvts"), lc=list("tpvb"))
+        C0 = rng.standard_normal((chip, d, w, chi))
+        check_contract(itg, T1, list("pwsb"), W, list("wvts"), lc=list("ptvb"), alpha=0.5,
+                       beta=-2.0, C0=C0)
+    for mode in ["fp32", "bf16x3"]:
+        T1 = rng.standard_normal((97, w, d, 131))
+        check_contract(itg, T1, list("pwsb"), rng.standard_normal((w, w, d, d)), list("wvts"),
+                       lc=list("ptvb"), mode=mode)
+
+
 def test_contract_config2_lpsi(itg):
     """Effective-Hamiltonian step 1 at chi=512: L(chi,w,chi) . psi(chi,d,d,chi)."""
     rng = np.random.default_rng(21)
