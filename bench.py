@@ -647,8 +647,8 @@ def main():
         roof_heff = roofline_from(kern[kname], "mfma", PEAK[head], "TFLOP/s",
                                   "every launch of the timed sweeps")
         roof_heff["kernel"] = kname
-        tpath = os.path.join(ROOT, "profiles", "gemm_split_kernel_traffic_chi2048.json")
-        if kname.startswith("gemm_split_kernel") and os.path.exists(tpath):
+        tpath = os.path.join(ROOT, "profiles", "gemm_split_wide_kernel_traffic_chi2048.json")
+        if kname.startswith("gemm_split") and os.path.exists(tpath):
             tr = json.load(open(tpath))
             roof_heff["traffic"] = tr["bytes_per_launch"]
             roof_heff["traffic_shape"] = tr["shape"]

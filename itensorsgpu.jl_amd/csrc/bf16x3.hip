@@ -11,6 +11,8 @@
 
 #include <algorithm>
 #include <cstdint>
+#include <cstdlib>
+#include <cstring>
 #include <type_traits>
 
 #include "bf16x3.h"
@@ -213,7 +215,166 @@ __global__ __launch_bounds__(256, 2) void gemm_split_kernel(
   }
 }
 
+// ------------------------------------------------------------ wide GEMM
+// The same product on 256 x 256 tiles: one 512-thread workgroup per CU,
+// eight waves of 64 (m) x 128 (n), K staged 32 deep through two LDS buffers
+// (each 4 planes x 256 rows x 32 bf16; 128 KB in all), one barrier per
+// stage. Against the 128 x 128 kernel a wave reads a third fewer LDS bytes
+// per MFMA (24 fragment loads per 96 products instead of 16 per 48) and the
+// workgroup stages half the operand bytes per product.
+//
+// A row is 64 bytes, four 16-byte k-chunks, stored at chunk c ^ sw(row) with
+// sw(row) = ((row >> 1) ^ (row >> 3)) & 3. A fragment load (16 rows, one
+// logical chunk) then covers every bank once per 8 lanes on 32 banks and
+// per 16 lanes on 64, and so does a staging store (4 rows x 4 chunks).
+constexpr int SW_M = 256, SW_N = 256, SW_K = 32, SW_LD = SW_K;
+constexpr int SW_PLANE = 256 * SW_LD;
+constexpr int SW_STAGE = 4 * SW_PLANE;
+constexpr size_t SW_LDS = 2 * (size_t)SW_STAGE * sizeof(uint16_t);
+
+__device__ __forceinline__ int wide_sw(int row) { return ((row >> 1) ^ (row >> 3)) & 3; }
+
+template <typename T>
+__global__ __launch_bounds__(512) void gemm_split_wide_kernel(
+    const uint16_t* __restrict__ Ah, const uint16_t* __restrict__ Al, int64_t lda,
+    const uint16_t* __restrict__ Bh, const uint16_t* __restrict__ Bl, int64_t ldb, int64_t M,
+    int64_t N, int64_t Kp, T* __restrict__ C, int64_t ldc, double alpha, double beta,
+    int64_t tilesM, int64_t tilesN) {
+  extern __shared__ __attribute__((aligned(16))) uint16_t lds[];
+
+  // XCD-aware grouped tile order (8 XCDs, groups of 8 tile rows)
+  int64_t nb = (int64_t)gridDim.x, bid = blockIdx.x;
+  int64_t lid = (nb % 8 == 0) ? (bid % 8) * (nb / 8) + bid / 8 : bid;
+  const int64_t GM = 8;
+  int64_t per_group = GM * tilesN;
+  int64_t group = lid / per_group;
+  int64_t first = group * GM;
+  int64_t gsize = std::min<int64_t>(tilesM - first, GM);
+  int64_t in = lid % per_group;
+  const int64_t m0 = (first + in % gsize) * SW_M;
+  const int64_t n0 = (in / gsize) * SW_N;
+
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  const int wm = wave & 3, wn = wave >> 2;
+  const int lr = lane & 15, lg = lane >> 4;
+
+  // loader: rows t / 4 and t / 4 + 128 of each operand, k-chunk (t % 4) * 8
+  const int lrow = t >> 2, lkc = (t & 3) * 8;
+  const int64_t ar0 = std::min<int64_t>(m0 + lrow, M - 1) * lda + lkc;
+  const int64_t ar1 = std::min<int64_t>(m0 + lrow + 128, M - 1) * lda + lkc;
+  const int64_t br0 = std::min<int64_t>(n0 + lrow, N - 1) * ldb + lkc;
+  const int64_t br1 = std::min<int64_t>(n0 + lrow + 128, N - 1) * ldb + lkc;
+  // the swizzle reads row bits 1..4, so rows lrow and lrow + 128 share it
+  const int so0 = lrow * SW_LD + (((t & 3) ^ wide_sw(lrow)) * 8), so1 = so0 + 128 * SW_LD;
+  uint4 ah0, ah1, al0, al1, bh0, bh1, bl0, bl1;
+#define WIDE_LOAD(k0)                                                               \
+  {                                                                                 \
+    ah0 = *(const uint4*)(Ah + ar0 + (k0)); ah1 = *(const uint4*)(Ah + ar1 + (k0)); \
+    al0 = *(const uint4*)(Al + ar0 + (k0)); al1 = *(const uint4*)(Al + ar1 + (k0)); \
+    bh0 = *(const uint4*)(Bh + br0 + (k0)); bh1 = *(const uint4*)(Bh + br1 + (k0)); \
+    bl0 = *(const uint4*)(Bl + br0 + (k0)); bl1 = *(const uint4*)(Bl + br1 + (k0)); \
+  }
+#define WIDE_STORE(s)                                                               \
+  {                                                                                 \
+    uint16_t* st = lds + (s) * SW_STAGE;                                            \
+    *(uint4*)(st + so0) = ah0; *(uint4*)(st + so1) = ah1;                           \
+    *(uint4*)(st + SW_PLANE + so0) = al0; *(uint4*)(st + SW_PLANE + so1) = al1;     \
+    *(uint4*)(st + 2 * SW_PLANE + so0) = bh0; *(uint4*)(st + 2 * SW_PLANE + so1) = bh1; \
+    *(uint4*)(st + 3 * SW_PLANE + so0) = bl0; *(uint4*)(st + 3 * SW_PLANE + so1) = bl1; \
+  }
+
+  f4_t acc[4][8];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 8; ++j) acc[i][j] = f4_t{0.f, 0.f, 0.f, 0.f};
+
+  const int64_t nk = Kp / SW_K;
+  WIDE_LOAD(0);
+  WIDE_STORE(0);
+  __syncthreads();
+  for (int64_t kt = 0; kt < nk; ++kt) {
+    const int s = (int)(kt & 1);
+    if (kt + 1 < nk) { WIDE_LOAD((kt + 1) * SW_K); }
+    const uint16_t* st = lds + s * SW_STAGE;
+    // Fragments are read one step ahead of the products that use them and
+    // in use order (the scheduling barriers keep that order), so that after
+    // the stage barrier a wave's first products wait for four reads rather
+    // than for the whole set while all eight waves read at once.
+    bf8_t ah[4], al[4], bh[2], bl[2];
+    auto read_a = [&](int i) {
+      const int row = wm * 64 + i * 16 + lr;
+      const int ma = row * SW_LD + ((lg ^ wide_sw(row)) * 8);
+      ah[i] = *(const bf8_t*)(st + ma);
+      al[i] = *(const bf8_t*)(st + SW_PLANE + ma);
+    };
+    auto read_b = [&](int j) {
+      const int row = wn * 128 + j * 16 + lr;
+      const int nb_ = row * SW_LD + ((lg ^ wide_sw(row)) * 8);
+      bh[j & 1] = *(const bf8_t*)(st + 2 * SW_PLANE + nb_);
+      bl[j & 1] = *(const bf8_t*)(st + 3 * SW_PLANE + nb_);
+    };
+    read_b(0);
+    read_a(0);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      if (i < 3) read_a(i + 1); else read_b(1);
+      __builtin_amdgcn_sched_barrier(0);
+      acc[i][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh[0], al[i], acc[i][0], 0, 0, 0);
+      acc[i][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bl[0], ah[i], acc[i][0], 0, 0, 0);
+      acc[i][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh[0], ah[i], acc[i][0], 0, 0, 0);
+    }
+#pragma unroll
+    for (int j = 1; j < 8; ++j) {
+      if (j < 7) read_b(j + 1);
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh[j & 1], al[i], acc[i][j], 0, 0, 0);
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bl[j & 1], ah[i], acc[i][j], 0, 0, 0);
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh[j & 1], ah[i], acc[i][j], 0, 0, 0);
+      }
+    }
+    // the other buffer was last read before the previous barrier
+    if (kt + 1 < nk) { WIDE_STORE(s ^ 1); }
+    __syncthreads();
+  }
+#undef WIDE_LOAD
+#undef WIDE_STORE
+
+  // lane holds D[n = 4 lg + r][m = lr] of block (i along m, j along n)
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    int64_t m = m0 + wm * 64 + i * 16 + lr;
+    if (m >= M) continue;
+#pragma unroll
+    for (int j = 0; j < 8; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        int64_t n = n0 + wn * 128 + j * 16 + 4 * lg + r;
+        if (n >= N) continue;
+        T* c = C + m + n * ldc;
+        double v = alpha * (double)acc[i][j][r];
+        if (beta != 0.0) v += beta * (double)*c;
+        *c = (T)v;
+      }
+  }
+}
+
 bool lds_ready = false;
+bool wide_ok = false;
+
+// One round of the wide kernel (one 256 x 256 tile per CU) takes as long as
+// ~1.66 rounds of the 128 x 128 kernel (two tiles per CU) at the same K
+// (MI355X, K = 2048 and 10240), so the wide tile is chosen when its rounds,
+// so weighted, are fewer: C1 and C4 at chi = 2048 (5 against 10 and 1 against
+// 2 rounds) but not C1 at chi = 1024 (2 against 3).
+bool use_wide(int64_t tiles_wide, int64_t tiles_narrow, int num_cu) {
+  if (num_cu <= 0) num_cu = 256;
+  int64_t rw = (tiles_wide + num_cu - 1) / num_cu;
+  int64_t rn = (tiles_narrow + 2 * num_cu - 1) / (2 * num_cu);
+  return 5 * rw < 3 * rn;
+}
 
 }  // namespace
 
@@ -250,7 +411,37 @@ int itg_gemm_split(itg_ctx* ctx, int dtype, int64_t M, int64_t N, int64_t K, int
                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     ITG_HIP(ctx, hipFuncSetAttribute((const void*)gemm_split_kernel<float>,
                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    // the wide tile needs 128 KB of a CU's 160 KB of LDS; without it the
+    // 128 x 128 kernel serves every shape
+    wide_ok = getenv("ITG_SPLIT_WIDE") == nullptr || strcmp(getenv("ITG_SPLIT_WIDE"), "0") != 0;
+    if (wide_ok &&
+        (hipFuncSetAttribute((const void*)gemm_split_wide_kernel<double>,
+                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)SW_LDS) != hipSuccess ||
+         hipFuncSetAttribute((const void*)gemm_split_wide_kernel<float>,
+                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)SW_LDS) != hipSuccess)) {
+      (void)hipGetLastError();
+      wide_ok = false;
+    }
     lds_ready = true;
+  }
+  const int64_t wM = (M + SW_M - 1) / SW_M, wN = (N + SW_N - 1) / SW_N;
+  if (wide_ok && use_wide(wM * wN, ((M + SB_M - 1) / SB_M) * ((N + SB_N - 1) / SB_N),
+                          ctx->num_cu)) {
+    if (wM * wN > 0x7fffffff) return itg_fail(ctx, ITG_EUNSUPPORTED, "split gemm: too large");
+    itg_tev tk = itg_tevents(ctx, ITG_K_GEMM_SPLIT);
+    if (dtype == ITG_F64)
+      hipExtLaunchKernelGGL(gemm_split_wide_kernel<double>, dim3((unsigned)(wM * wN)), dim3(512),
+                            (uint32_t)SW_LDS, ctx->stream, tk.a, tk.b, 0, Ah, Al, lda, Bh, Bl,
+                            ldb, M, N, Kp, (double*)C, ldc, alpha, beta, wM, wN);
+    else
+      hipExtLaunchKernelGGL(gemm_split_wide_kernel<float>, dim3((unsigned)(wM * wN)), dim3(512),
+                            (uint32_t)SW_LDS, ctx->stream, tk.a, tk.b, 0, Ah, Al, lda, Bh, Bl,
+                            ldb, M, N, Kp, (float*)C, ldc, alpha, beta, wM, wN);
+    ITG_LAUNCH_CHECK(ctx);
+    double es = (double)itg_dtype_size(dtype);
+    itg_tcommit(ctx, tk, ITG_K_GEMM_SPLIT, 2.0 * M * N * K,
+                4.0 * (M + N) * K + es * M * N * (beta != 0.0 ? 2.0 : 1.0));
+    return ITG_OK;
   }
   int64_t tilesM = (M + SB_M - 1) / SB_M, tilesN = (N + SB_N - 1) / SB_N;
   int64_t tiles = tilesM * tilesN;

@@ -21,7 +21,7 @@ TRUNC_NDTENSORS, TRUNC_REFGPU = 0, 1
 MODES = {"fp64": MODE_NATIVE, "native": MODE_NATIVE, "fp32": MODE_F32, "bf16x3": MODE_BF16X3}
 # kernel classes of itg_timing
 K_GEMM_SPLIT, K_GETT, K_WPASS, K_TRD_SYMV, K_TRD_POST = 0, 1, 2, 3, 4
-KERNEL_NAMES = {K_GEMM_SPLIT: "gemm_split_kernel (split-BF16 GEMM)",
+KERNEL_NAMES = {K_GEMM_SPLIT: "gemm_split_wide_kernel / gemm_split_kernel (split-BF16 GEMM)",
                 K_GETT: "contraction GEMM (GETT kernel or rocBLAS dgemm)",
                 K_WPASS: "heff_wpass_kernel (W1.W2 pass)",
                 K_TRD_SYMV: "trd_symv_kernel (tridiagonalisation symv)",

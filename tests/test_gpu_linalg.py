@@ -101,6 +101,28 @@ def test_heff2_config2_chi1024(dev):
         assert err < tol
 
 
+@pytest.mark.parametrize("chiL,w,d,chiR,dtype", [(800, 5, 2, 1000, "float64"),
+                                                 (800, 5, 2, 1000, "float32"),
+                                                 (500, 2, 4, 1900, "float64")])
+def test_heff2_bf16x3_large_ragged(dev, chiL, w, d, chiR, dtype):
+    """Split-BF16 products large enough for the 256 x 256 tile, ragged in both
+    directions: C1 at 4000 x 4000 (K = 800), and in the last case C1 at
+    1000 x 30400 (K = 500) and C4 at 8000 x 1900 (K = 3800), against the
+    oracle."""
+    rng = np.random.default_rng(chiL + chiR)
+    L = sym_env(rng, chiL, w)
+    R = sym_env(rng, chiR, w)
+    W1 = rng.standard_normal((w, w, d, d))
+    W2 = rng.standard_normal((w, w, d, d))
+    psi = rng.standard_normal((chiL, d, d, chiR))
+    want = odmrg.heff_apply_blas(L, W1, W2, R, psi)   # equal to heff_apply (test_oracle)
+    y = dev.heff2_apply(*(up(dev, a, dtype) for a in (L, W1, W2, R, psi)),
+                        chiL, w, w, w, d, chiR, mode="bf16x3")
+    err = rel(ops.from_flat(dev.to_host(y), want.shape), want)
+    print((chiL, w, d, chiR, dtype), err)
+    assert err < 5e-5
+
+
 # -------------------------------------------------------------------- eigen
 def density_matrix(rng, n, decades=14):
     lam = 10.0 ** (-np.linspace(0, decades, n)) * rng.uniform(0.5, 1.0, n)
