@@ -123,6 +123,27 @@ def test_heff2_bf16x3_large_ragged(dev, chiL, w, d, chiR, dtype):
     assert err < 5e-5
 
 
+def test_heff2_bf16x3_headline_shape_against_fp64(dev):
+    """The split-BF16 apply at the bulk-bond shape of the sweep benchmark
+    (chi=2048, w=5, d=2: C1 is 10240 x 8192 x 2048 and C4 8192 x 2048 x
+    10240, both on 256 x 256 tiles) against the FP64 apply of the same
+    device operands, which matches the oracle at chi=1024 above; the
+    tolerance is the oracle tests'."""
+    chi, w, d = 2048, 5, 2
+    L = dev.randn(chi * w * chi, 31)
+    R = dev.randn(chi * w * chi, 32)
+    W1 = dev.randn(w * w * d * d, 33)
+    W2 = dev.randn(w * w * d * d, 34)
+    x = dev.randn(chi * d * d * chi, 35)
+    y64 = dev.heff2_apply(L, W1, W2, R, x, chi, w, w, w, d, chi, mode="native")
+    y3 = dev.heff2_apply(L, W1, W2, R, x, chi, w, w, w, d, chi, mode="bf16x3")
+    ref = dev.nrm2(y64)
+    dev.axpby(-1.0, y64, 1.0, y3)
+    err = dev.nrm2(y3) / ref
+    print("chi=2048 bf16x3 against fp64", err)
+    assert 0.0 < err < 5e-5
+
+
 # -------------------------------------------------------------------- eigen
 def density_matrix(rng, n, decades=14):
     lam = 10.0 ** (-np.linspace(0, decades, n)) * rng.uniform(0.5, 1.0, n)
