@@ -73,6 +73,43 @@ __global__ __launch_bounds__(256) void heff_wpass_kernel(int64_t chiL, int64_t Q
   }
 }
 
+// The same pass for fixed Q x O (the chain's w = 5, d = 2: 20 x 20): one
+// thread per row a' for WPASS_B consecutive b, T1(a', q, b) loaded and
+// T3(a', o, b) stored along a' (256 consecutive rows per block), X read from
+// LDS by broadcast. The sum over q runs in the order of heff_wpass_kernel.
+constexpr int WPASS_B = 4;
+
+template <typename T, int QC, int OC>
+__global__ __launch_bounds__(256) void heff_wpass_fixed_kernel(int64_t chiL, int64_t chiR,
+                                                               const T* __restrict__ T1,
+                                                               const double* __restrict__ X,
+                                                               T* __restrict__ T3) {
+  __shared__ double xs[QC * OC];
+  for (int e = threadIdx.x; e < QC * OC; e += 256) xs[e] = X[e];
+  __syncthreads();
+  const int64_t tiles_a = (chiL + 255) / 256;
+  const int64_t a = (blockIdx.x % tiles_a) * 256 + threadIdx.x;
+  const int64_t b0 = (blockIdx.x / tiles_a) * WPASS_B;
+  if (a >= chiL) return;
+  const int64_t b1 = b0 + WPASS_B < chiR ? b0 + WPASS_B : chiR;
+  for (int64_t b = b0; b < b1; ++b) {
+    const T* src = T1 + (int64_t)QC * chiL * b + a;
+    double t[QC];
+#pragma unroll
+    for (int q = 0; q < QC; ++q) t[q] = (double)src[chiL * q];
+    T* dst = T3 + (int64_t)OC * chiL * b + a;
+    // a few columns of X at a time: unrolled whole, the 400 loads would be
+    // hoisted out of the b loop and spill
+#pragma unroll 2
+    for (int o = 0; o < OC; ++o) {
+      double acc = 0.0;
+#pragma unroll
+      for (int q = 0; q < QC; ++q) acc += t[q] * xs[q + QC * o];
+      dst[chiL * o] = (T)acc;
+    }
+  }
+}
+
 template <typename T>
 __global__ void heff_diag_kernel(int64_t chiL, int64_t WL, int64_t WM, int64_t WR, int64_t d,
                                  int64_t chiR, const T* L, const T* W1, const T* W2, const T* R,
@@ -222,12 +259,21 @@ int apply(itg_ctx* ctx, int dtype, int mode, const HeffDims& h, const void* L, c
   int64_t blocks = ((h.chiL + 63) / 64) * h.chiR;
   size_t lds = (size_t)(h.Q * 64 + h.Q * h.O) * sizeof(double);
   int tk = itg_tstart(ctx, ITG_K_WPASS);
-  if (idt == ITG_F64)
+  if (h.Q == 20 && h.O == 20) {
+    const int64_t fb = ((h.chiL + 255) / 256) * ((h.chiR + WPASS_B - 1) / WPASS_B);
+    if (idt == ITG_F64)
+      heff_wpass_fixed_kernel<double, 20, 20><<<(unsigned)fb, 256, 0, ctx->stream>>>(
+          h.chiL, h.chiR, (const double*)w.T1, P.X, (double*)w.T3);
+    else
+      heff_wpass_fixed_kernel<float, 20, 20><<<(unsigned)fb, 256, 0, ctx->stream>>>(
+          h.chiL, h.chiR, (const float*)w.T1, P.X, (float*)w.T3);
+  } else if (idt == ITG_F64) {
     heff_wpass_kernel<double><<<(unsigned)blocks, 256, lds, ctx->stream>>>(
         h.chiL, h.Q, h.O, (const double*)w.T1, P.X, (double*)w.T3);
-  else
+  } else {
     heff_wpass_kernel<float><<<(unsigned)blocks, 256, lds, ctx->stream>>>(
         h.chiL, h.Q, h.O, (const float*)w.T1, P.X, (float*)w.T3);
+  }
   ITG_LAUNCH_CHECK(ctx);
   itg_tstop(ctx, tk, ITG_K_WPASS, 2.0 * h.Q * h.O * h.chiL * h.chiR,
             (double)(h.Q + h.O) * h.chiL * h.chiR * itg_dtype_size(idt));
